@@ -1,0 +1,190 @@
+/*
+ * pbbs_bfs.h -- C ABI of the MI355X (gfx950) breadth-first search.
+ *
+ * Drop-in for PBBS breadthFirstSearch/deterministicBFS.  The reference binds
+ * its implementation at link time through one C++ function
+ *
+ *     parlay::sequence<vertexId> BFS(vertexId start, const Graph& G, bool verbose)
+ *     (benchmarks/breadthFirstSearch/bench/BFS.h:4-11,
+ *      defined benchmarks/breadthFirstSearch/deterministicBFS/BFS.C:39-78)
+ *
+ * and reads its graph through readGraphFromFile (common/graphIO.h:191-215)
+ * into graph<int,uint> (common/graph.h:104-152).  This header replaces that
+ * pair with plain C entry points: a context (one GPU), a graph resident in
+ * HBM, and BFS calls whose outputs are bit-identical to deterministicBFS:
+ *
+ *     parent[s] = s, parent[v] = -1 if v is unreached, otherwise
+ *     parent[v] = the LARGEST vertex id u with (u,v) an edge and
+ *                 depth(u) = depth(v) - 1
+ *     (the write_max claim of deterministicBFS/BFS.C:65-68 with
+ *      common/atomics.h:75-81, followed by the winner filter BFS.C:71-74)
+ *
+ *     depth[s] = 0, depth[v] = n if v is unreached, otherwise the hop count
+ *     (the level_number convention of bench/BFSCheck.C:39-56).
+ *
+ * Conventions: every function returns PBBS_OK (0) or a negative PBBS_ERR_*
+ * code; pbbs_last_error() gives a message for the calling thread.  No C++
+ * exception crosses this boundary.  A context is used by one host thread at a
+ * time (as the reference's BFS() is called from one thread,
+ * bench/BFSTime.C:38-41); the server object serialises its own work.
+ * Vertex ids are int32 (the reference's signed vertexId, bench/BFS.h:4);
+ * offsets are uint32 (the reference's edgeId = uint, bench/BFS.h:6) or int64.
+ */
+#ifndef PBBS_BFS_H_
+#define PBBS_BFS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PBBS_OK              0
+#define PBBS_ERR_INVALID    -1  /* bad argument (null pointer, source out of range, ...) */
+#define PBBS_ERR_HIP        -2  /* a HIP runtime call failed */
+#define PBBS_ERR_NOMEM      -3  /* device or host allocation failed */
+#define PBBS_ERR_LIMIT      -4  /* size beyond a documented limit */
+#define PBBS_ERR_FORMAT     -5  /* malformed input file */
+#define PBBS_ERR_IO         -6  /* file could not be opened / written */
+#define PBBS_ERR_STATE      -7  /* call not valid in the object's current state */
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* pbbs_last_error(void);
+
+/* Library version string, e.g. "pbbs-bfs-gfx950 0.1". */
+const char* pbbs_version(void);
+
+/* ---- context ----------------------------------------------------------- */
+typedef struct pbbs_ctx pbbs_ctx;
+
+/* Opens `device` (HIP ordinal), creates the context's stream.  Replaces the
+ * implicit global parlay scheduler the reference's BFS() runs on
+ * (common/parallelDefs:11-28). */
+int pbbs_ctx_create(int device, pbbs_ctx** out);
+int pbbs_ctx_destroy(pbbs_ctx* ctx);
+
+/* Direction-optimising switch and launch shape.  Defaults follow ligraLight's
+ * edge_map rule (common/ligraLight.h:120-133): go bottom-up when
+ * |F| + sum deg(F) > m / td_to_bu_div, stay bottom-up while |F| > n / bu_to_td_div. */
+typedef struct {
+    int32_t td_to_bu_div;   /* default 20; 0 disables bottom-up entirely */
+    int32_t bu_to_td_div;   /* default 20 */
+    int32_t grid_blocks;    /* workgroups per level launch; 0 = auto (occupancy x CUs) */
+    int32_t bu_serial_depth;/* neighbours a lane tests alone before its wavefront helps; default 4 */
+    int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16 */
+    int32_t use_graphs;     /* 1 = replay captured HIP graphs of level launches (default),
+                               0 = launch each level kernel on the stream */
+    int32_t reserved[2];
+} pbbs_tuning;
+
+int pbbs_tuning_get(const pbbs_ctx* ctx, pbbs_tuning* out);
+int pbbs_tuning_set(pbbs_ctx* ctx, const pbbs_tuning* t);
+
+/* ---- graph resident in HBM --------------------------------------------- */
+typedef struct pbbs_graph pbbs_graph;
+
+#define PBBS_GRAPH_OFFSETS_I64      0x1u  /* offsets are int64_t[n+1] (else uint32_t[n+1]);
+                                             HBM keeps the width given */
+#define PBBS_GRAPH_DEVICE_INPUT     0x2u  /* offsets/adj point to device memory */
+#define PBBS_GRAPH_ASSUME_SYMMETRIC 0x8u  /* skip the symmetry check; caller vouches for it */
+
+typedef struct {
+    int64_t n;              /* vertices */
+    int64_t m;              /* directed CSR entries */
+    int32_t offsets_i64;    /* 1 if HBM offsets are int64 */
+    int32_t sorted;         /* 1 if every adjacency row is ascending */
+    int32_t symmetric;      /* 1 if (u,v) in E <=> (v,u) in E (checked on upload) */
+    int32_t pad;
+    int64_t isolated;       /* vertices of degree 0 */
+    int64_t max_degree;
+    int64_t device_bytes;   /* HBM held by the graph */
+} pbbs_graph_info;
+
+/* Copies a CSR graph (`offsets[n+1]`, offsets[n] == m, `adj[m]`) into HBM and
+ * computes the per-graph metadata (row sortedness, symmetry, isolated-vertex
+ * bitmap).  Replaces graph<int,uint>'s construction in readGraphFromFile
+ * (common/graphIO.h:191-215, common/graph.h:141-148).  Limits:
+ * 0 < n < 2^31, m < 2^32.  Bottom-up steps need a symmetric graph (they scan
+ * N(v) as in-neighbours); a non-symmetric graph runs top-down only, as
+ * deterministicBFS does on a directed graph. */
+int pbbs_graph_upload(pbbs_ctx* ctx, int64_t n, int64_t m, const void* offsets,
+                      const int32_t* adj, uint32_t flags, pbbs_graph** out);
+int pbbs_graph_info_get(const pbbs_graph* g, pbbs_graph_info* out);
+int pbbs_graph_free(pbbs_graph* g);
+
+/* ---- single-source BFS ------------------------------------------------- */
+#define PBBS_OUT_DEVICE   0x1u  /* parent/depth are device pointers (no D2H copy) */
+
+typedef struct {
+    int32_t levels;         /* BFS levels executed (max depth + 1) */
+    int32_t td_levels;      /* levels run top-down */
+    int32_t bu_levels;      /* levels run bottom-up */
+    int32_t launches;       /* level-kernel launches, including no-op overshoot */
+    int64_t reached;        /* n_r: vertices with parent != -1 */
+    int64_t edges_reached;  /* m_r: sum of degrees over reached vertices */
+    double  device_ms;      /* hipEvent time: init kernel start -> last level done */
+    double  copy_ms;        /* D2H time of the outputs (0 with PBBS_OUT_DEVICE) */
+} pbbs_bfs_stats;
+
+/* One BFS from `source`.  parent[n] and depth[n] (either may be NULL) receive
+ * the results above; with PBBS_OUT_DEVICE they are device pointers and the
+ * call returns once they are written in HBM.  Replaces
+ * BFS(start, G, verbose) of bench/BFS.h:11 / deterministicBFS/BFS.C:39. */
+int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
+             int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats);
+
+/* ---- host-side CSR container, generators and PBBS text I/O ------------- */
+/* A host CSR: int64 offsets[n+1], int32 adj[m], rows ascending, no duplicate
+ * entries.  Mirrors graph<intV,intE> (common/graph.h:104-152). */
+typedef struct {
+    int64_t  n;
+    int64_t  m;
+    int64_t* offsets;
+    int32_t* adj;
+} pbbs_csr;
+
+void pbbs_csr_free(pbbs_csr* g);
+
+/* Symmetric RMAT graph: 2^scale vertices, `draws` directed edge draws with
+ * quadrant probabilities a, b, c (d = 1-a-b-c), seeded; self-loops dropped,
+ * reverse edges added, duplicates removed, rows sorted.  The recursion
+ * follows testData/graphData/rMatGraph.C:53-79 (one uniform draw per level
+ * picks the quadrant) with this library's own counter-based hash in place of
+ * parlay::hash64.  `permute_seed` != 0 relabels vertices with a seeded random
+ * permutation (the unordered variant, common/graphUtils.h:166-189). */
+int pbbs_gen_rmat(int32_t scale, int64_t draws, double a, double b, double c,
+                  uint64_t seed, uint64_t permute_seed, pbbs_csr* out);
+
+/* d x d x d torus, degree 6: the same graph as `gridGraph -j -d 3 -o d^3`
+ * (testData/graphData/gridGraph.C:53-72, symmetrised). */
+int pbbs_gen_grid3d(int32_t d, pbbs_csr* out);
+
+/* Random-local graph: vertex i draws `k` neighbours j = (i +- floor(n^u)) mod n,
+ * u ~ U[0,1), random sign; self-loops dropped, symmetrised, deduplicated. */
+int pbbs_gen_randlocal(int64_t n, int32_t k, uint64_t seed, pbbs_csr* out);
+
+/* Builds a CSR from an edge list (u[i], v[i]); symmetrise != 0 adds reverse
+ * edges.  Self-loops are dropped when symmetrising (common/graphUtils.h:63-71);
+ * duplicates are always removed (common/graphUtils.h:54-61). */
+int pbbs_csr_from_edges(int64_t n, int64_t num_edges, const int32_t* u,
+                        const int32_t* v, int symmetrise, pbbs_csr* out);
+
+/* PBBS AdjacencyGraph text format (docs/fileFormats/graph.md:16-39;
+ * reader common/graphIO.h:191-215, writer common/graphIO.h:78-105).  Rows
+ * are kept in file order. */
+int pbbs_read_adjacency_graph(const char* path, pbbs_csr* out);
+int pbbs_write_adjacency_graph(const char* path, const pbbs_csr* g);
+
+/* PBBS sequenceInt text format (docs/fileFormats/sequence.md:10-26; writer
+ * common/IO.h:151-166 via common/sequenceIO.h:189-193, reader
+ * common/IO.h:217-229).  The reader allocates *out with malloc. */
+int pbbs_write_sequence_int(const char* path, const int32_t* a, int64_t n);
+int pbbs_read_sequence_int(const char* path, int32_t** out, int64_t* n);
+void pbbs_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBBS_BFS_H_ */

@@ -1,0 +1,115 @@
+// pbbs_bfs.hpp -- C++ face of the C ABI in pbbs_bfs.h, shaped like the
+// reference's benchmark interface (benchmarks/breadthFirstSearch/bench/BFS.h:4-11):
+//
+//     using vertexId = int;  using edgeId = uint;
+//     sequence<vertexId> BFS(vertexId start, const Graph& G, bool verbose);
+//
+// Here Graph is a CSR resident in HBM (built from a host CSR once, as the
+// reference builds graph<int,uint> once in bench/BFSTime.C:58-59), and BFS()
+// returns the parent sequence in host memory.  Errors become exceptions on
+// this side of the boundary only.
+#pragma once
+
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pbbs_bfs.h"
+
+namespace pbbs_amd {
+
+using vertexId = int;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+inline void check(int rc, const char* what) {
+  if (rc != PBBS_OK) throw Error(rc, std::string(what) + ": " + pbbs_last_error());
+}
+
+// Owning host CSR (int64 offsets, int32 ids).
+class HostGraph {
+ public:
+  HostGraph() = default;
+  explicit HostGraph(const pbbs_csr& g) : g_(g) {}
+  HostGraph(const HostGraph&) = delete;
+  HostGraph& operator=(const HostGraph&) = delete;
+  HostGraph(HostGraph&& o) noexcept : g_(o.g_) { o.g_ = pbbs_csr{}; }
+  HostGraph& operator=(HostGraph&& o) noexcept {
+    std::swap(g_, o.g_);
+    return *this;
+  }
+  ~HostGraph() { pbbs_csr_free(&g_); }
+
+  static HostGraph read(const char* path) {  // common/graphIO.h:191-215
+    pbbs_csr g{};
+    check(pbbs_read_adjacency_graph(path, &g), "readGraphFromFile");
+    return HostGraph(g);
+  }
+  size_t numVertices() const { return (size_t)g_.n; }
+  size_t numEdges() const { return (size_t)g_.m; }
+  const pbbs_csr& raw() const { return g_; }
+
+ private:
+  pbbs_csr g_{};
+};
+
+class Context {
+ public:
+  explicit Context(int device = 0) { check(pbbs_ctx_create(device, &c_), "pbbs_ctx_create"); }
+  Context(const Context&) = delete;
+  Context& operator=(const Context&) = delete;
+  ~Context() { pbbs_ctx_destroy(c_); }
+  pbbs_ctx* get() const { return c_; }
+
+ private:
+  pbbs_ctx* c_ = nullptr;
+};
+
+// Graph resident in HBM.  `wide_offsets` keeps int64 offsets on the device;
+// otherwise offsets are narrowed to uint32 (the reference's edgeId).
+class Graph {
+ public:
+  Graph(Context& ctx, const HostGraph& h, bool wide_offsets = false) : n_(h.numVertices()) {
+    const pbbs_csr& g = h.raw();
+    if (wide_offsets) {
+      check(pbbs_graph_upload(ctx.get(), g.n, g.m, g.offsets, g.adj, PBBS_GRAPH_OFFSETS_I64, &g_),
+            "pbbs_graph_upload");
+    } else {
+      std::vector<uint32_t> off32(g.n + 1);
+      for (int64_t i = 0; i <= g.n; ++i) off32[i] = (uint32_t)g.offsets[i];
+      check(pbbs_graph_upload(ctx.get(), g.n, g.m, off32.data(), g.adj, 0, &g_),
+            "pbbs_graph_upload");
+    }
+    ctx_ = ctx.get();
+  }
+  Graph(const Graph&) = delete;
+  Graph& operator=(const Graph&) = delete;
+  ~Graph() { pbbs_graph_free(g_); }
+  size_t numVertices() const { return n_; }
+  pbbs_graph* get() const { return g_; }
+  pbbs_ctx* ctx() const { return ctx_; }
+
+ private:
+  pbbs_graph* g_ = nullptr;
+  pbbs_ctx* ctx_ = nullptr;
+  size_t n_ = 0;
+};
+
+// Parent sequence: -1 = unvisited, the start points to itself
+// (bench/BFS.h:9-11).  `depth`, if given, receives the level_number depths.
+inline std::vector<vertexId> BFS(vertexId start, const Graph& G, bool verbose = false,
+                                 std::vector<vertexId>* depth = nullptr,
+                                 pbbs_bfs_stats* stats = nullptr) {
+  (void)verbose;
+  std::vector<vertexId> parent(G.numVertices());
+  if (depth) depth->resize(G.numVertices());
+  check(pbbs_bfs(G.ctx(), G.get(), start, parent.data(), depth ? depth->data() : nullptr, 0, stats),
+        "BFS");
+  return parent;
+}
+
+}  // namespace pbbs_amd

@@ -1,0 +1,261 @@
+/*
+ * ORACLE -- test infrastructure only.  Nothing in pbbsbench_amd/ links,
+ * loads or calls this file; only tests/, __graft_entry__.smoke() and the
+ * CPU-baseline leg of bench.py use it, as the checker.
+ *
+ * A CPU restatement of the reference path (PBBS breadthFirstSearch/
+ * deterministicBFS and its checker).  The reference itself cannot be built:
+ * its parlaylib submodule is empty.  Functions:
+ *
+ *   oracle_det_bfs      deterministicBFS, benchmarks/breadthFirstSearch/
+ *                       deterministicBFS/BFS.C:39-78, with the write_max of
+ *                       common/atomics.h:75-81; OpenMP threads.
+ *   oracle_level_number bench/BFSCheck.C:39-56 (pointer doubling).
+ *   oracle_check_bfs    bench/BFSCheck.C:59-90 with the source as a parameter
+ *                       (the reference hard-codes 0 at bench/BFSCheck.C:101).
+ *   oracle_serial_bfs   independent brute force of the closed form the
+ *                       restatement must meet: FIFO levels (as serialBFS/
+ *                       BFS.C:32-55 computes them), then parent[v] = the
+ *                       largest u with (u,v) in E one level above v.
+ *   oracle_max_rule     counts vertices violating that closed form.
+ *
+ * Graphs are CSR with int64 offsets[n+1] and int32 adj[m].
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+/* common/atomics.h:75-81: CAS loop that stores b while the current value is
+ * less than b; returns 1 iff this call wrote. */
+static inline int write_max(int32_t* a, int32_t b) {
+    int32_t c = __atomic_load_n(a, __ATOMIC_RELAXED);
+    while (c < b) {
+        if (__atomic_compare_exchange_n(a, &c, b, 0, __ATOMIC_SEQ_CST, __ATOMIC_RELAXED))
+            return 1;
+    }
+    return 0;
+}
+
+typedef struct { int32_t u, v; } pair32;
+typedef struct { pair32* p; int64_t n, cap; } pairbuf;
+
+static int push(pairbuf* b, int32_t u, int32_t v) {
+    if (b->n == b->cap) {
+        int64_t cap = b->cap ? 2 * b->cap : 1024;
+        pair32* np = (pair32*)realloc(b->p, (size_t)cap * sizeof(pair32));
+        if (!np) return -1;
+        b->p = np;
+        b->cap = cap;
+    }
+    b->p[b->n].u = u;
+    b->p[b->n].v = v;
+    b->n++;
+    return 0;
+}
+
+#define EDGE_CHUNK 4096
+
+/* Returns the number of rounds (levels), or -1 on allocation failure.
+ * parent[n] receives the result: -1 unreached, parent[s] = s. */
+int64_t oracle_det_bfs(int64_t n, const int64_t* off, const int32_t* adj, int32_t s,
+                       int32_t* parent, int nthreads) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    uint8_t* visited = (uint8_t*)calloc((size_t)n, 1);
+    int32_t* frontier = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    int32_t* next = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    int64_t* chunk_prefix = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
+    pairbuf* kept = (pairbuf*)calloc((size_t)nthreads, sizeof(pairbuf));
+    int64_t* emit = (int64_t*)calloc((size_t)nthreads + 1, sizeof(int64_t));
+    int64_t rounds = -1;
+    if (!visited || !frontier || !next || !chunk_prefix || !kept || !emit) goto done;
+
+    /* BFS.C:41-50 */
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) parent[i] = -1;
+    parent[s] = s;
+    visited[s] = 1;
+    frontier[0] = s;
+    int64_t fsize = 1;
+    rounds = 0;
+    int oom = 0;
+    while (fsize > 0) { /* BFS.C:54 */
+        rounds++;
+        /* BFS.C:59-62: the frontier's out-edges, flattened.  Rows are cut into
+         * EDGE_CHUNK pieces so one high-degree vertex spreads over threads. */
+        chunk_prefix[0] = 0;
+        for (int64_t i = 0; i < fsize; ++i) {
+            int64_t d = off[frontier[i] + 1] - off[frontier[i]];
+            chunk_prefix[i + 1] = chunk_prefix[i] + (d + EDGE_CHUNK - 1) / EDGE_CHUNK;
+        }
+        const int64_t nchunks = chunk_prefix[fsize];
+#pragma omp parallel num_threads(nthreads)
+        {
+            const int t = omp_get_thread_num();
+            kept[t].n = 0;
+            /* BFS.C:65-68: keep (u,v) iff !visited[v] && write_max(&parent[v], u) */
+#pragma omp for schedule(dynamic, 16)
+            for (int64_t c = 0; c < nchunks; ++c) {
+                int64_t lo = 0, hi = fsize; /* chunk_prefix[lo] <= c < chunk_prefix[lo+1] */
+                while (hi - lo > 1) {
+                    int64_t mid = (lo + hi) / 2;
+                    if (chunk_prefix[mid] <= c) lo = mid; else hi = mid;
+                }
+                const int32_t u = frontier[lo];
+                const int64_t b = off[u] + (c - chunk_prefix[lo]) * EDGE_CHUNK;
+                const int64_t e = b + EDGE_CHUNK < off[u + 1] ? b + EDGE_CHUNK : off[u + 1];
+                for (int64_t j = b; j < e; ++j) {
+                    const int32_t v = adj[j];
+                    if (!__atomic_load_n(&visited[v], __ATOMIC_RELAXED) && write_max(&parent[v], u))
+                        if (push(&kept[t], u, v)) oom = 1;
+                }
+            }
+            /* implicit barrier: every claim of the round is done (the filter
+             * of BFS.C:65 is materialised before BFS.C:71 reads parent) */
+            /* BFS.C:71-74: keep the winners, mark them visited */
+            int64_t cnt = 0;
+            for (int64_t i = 0; i < kept[t].n; ++i)
+                if (__atomic_load_n(&parent[kept[t].p[i].v], __ATOMIC_RELAXED) == kept[t].p[i].u)
+                    kept[t].p[cnt++] = kept[t].p[i];
+            kept[t].n = cnt;
+            emit[t + 1] = cnt;
+#pragma omp barrier
+#pragma omp single
+            for (int i = 0; i < nthreads; ++i) emit[i + 1] += emit[i];
+            for (int64_t i = 0; i < kept[t].n; ++i) {
+                __atomic_store_n(&visited[kept[t].p[i].v], 1, __ATOMIC_RELAXED);
+                next[emit[t] + i] = kept[t].p[i].v;
+            }
+        }
+        if (oom) { rounds = -1; goto done; }
+        fsize = emit[nthreads];
+        int32_t* tmp = frontier; frontier = next; next = tmp;
+    }
+done:
+    if (kept) for (int i = 0; i < nthreads; ++i) free(kept[i].p);
+    free(kept); free(emit); free(visited); free(frontier); free(next); free(chunk_prefix);
+    return rounds;
+}
+
+/* bench/BFSCheck.C:39-56: (count, ptr) pointer doubling to a fixed point;
+ * unreached vertices get n.  Returns 0, -1 on OOM, or 1 if no fixed point is
+ * reached within 40 doublings (the parents contain a cycle; the reference
+ * would loop for ever there).  Counts are kept in 64 bits so a cycle cannot
+ * overflow them within the cap. */
+int oracle_level_number(int64_t n, const int32_t* parent, int32_t* level) {
+    int64_t* cnt = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    int32_t* ptr = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    int64_t* cnt2 = (int64_t*)malloc((size_t)n * sizeof(int64_t));
+    int32_t* ptr2 = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    if (!cnt || !ptr || !cnt2 || !ptr2) { free(cnt); free(ptr); free(cnt2); free(ptr2); return -1; }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        if (parent[i] == -1 || parent[i] == i) { cnt[i] = 0; ptr[i] = (int32_t)i; }
+        else { cnt[i] = 1; ptr[i] = parent[i]; }
+    }
+    int converged = 0;
+    for (int round = 0; round < 40 && !converged; ++round) {
+        int64_t changed = 0;
+#pragma omp parallel for schedule(static) reduction(+ : changed)
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t p = ptr[i];
+            cnt2[i] = cnt[p] + cnt[i];
+            ptr2[i] = ptr[p];
+            changed += (cnt2[i] != cnt[i]) || (ptr2[i] != ptr[i]);
+        }
+        int64_t* tc = cnt; cnt = cnt2; cnt2 = tc;
+        int32_t* tp = ptr; ptr = ptr2; ptr2 = tp;
+        converged = changed == 0;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) level[i] = (parent[ptr[i]] == -1) ? (int32_t)n : (int32_t)cnt[i];
+    free(cnt); free(ptr); free(cnt2); free(ptr2);
+    return converged ? 0 : 1;
+}
+
+/* bench/BFSCheck.C:59-90 with `start` as a parameter.  Returns 0 if the
+ * parents form a valid BFS tree for a symmetric graph, else 1 and the first
+ * failing vertex in *fail_vertex (-1 for a size mismatch); -1 on OOM. */
+int oracle_check_bfs(int64_t n, const int64_t* off, const int32_t* adj, int32_t start,
+                     const int32_t* parents, int64_t parents_n, int64_t* fail_vertex) {
+    *fail_vertex = -1;
+    if (n != parents_n) return 1;
+    for (int64_t v = 0; v < n; ++v)
+        if (parents[v] < -1 || parents[v] >= n) { *fail_vertex = v; return 1; }
+    int32_t* level = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    if (!level) return -1;
+    const int ln = oracle_level_number(n, parents, level);
+    if (ln != 0) { free(level); return ln < 0 ? -1 : 1; }
+    int64_t first_bad = n;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(min : first_bad)
+    for (int64_t v = 0; v < n; ++v) {
+        int ok;
+        if (v == start) {
+            ok = parents[v] == v;
+        } else {
+            int32_t mn = (int32_t)n; /* degree 0: level.size() */
+            for (int64_t j = off[v]; j < off[v + 1]; ++j)
+                if (level[adj[j]] < mn) mn = level[adj[j]];
+            ok = (level[v] == n && mn == n) || level[v] == mn + 1;
+        }
+        if (!ok && v < first_bad) first_bad = v;
+    }
+    free(level);
+    if (first_bad < n) { *fail_vertex = first_bad; return 1; }
+    return 0;
+}
+
+/* Closed form by brute force, serial.  depth: FIFO BFS levels, n if
+ * unreached.  parent: s for s, -1 if unreached, else the largest u with
+ * (u,v) in E and depth[u] == depth[v] - 1 (found by scanning every edge, so
+ * it also holds for directed graphs).  Returns 0, or -1 on OOM. */
+int oracle_serial_bfs(int64_t n, const int64_t* off, const int32_t* adj, int32_t s,
+                      int32_t* parent, int32_t* depth) {
+    int32_t* queue = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+    if (!queue) return -1;
+    for (int64_t i = 0; i < n; ++i) { depth[i] = (int32_t)n; parent[i] = -1; }
+    int64_t head = 0, tail = 0;
+    depth[s] = 0;
+    queue[tail++] = s;
+    while (head < tail) {
+        const int32_t u = queue[head++];
+        for (int64_t j = off[u]; j < off[u + 1]; ++j) {
+            const int32_t v = adj[j];
+            if (depth[v] == n) { depth[v] = depth[u] + 1; queue[tail++] = v; }
+        }
+    }
+    free(queue);
+    parent[s] = s;
+    for (int64_t u = 0; u < n; ++u) {
+        if (depth[u] == n) continue;
+        for (int64_t j = off[u]; j < off[u + 1]; ++j) {
+            const int32_t v = adj[j];
+            if (v != s && depth[v] == depth[u] + 1 && (int32_t)u > parent[v]) parent[v] = (int32_t)u;
+        }
+    }
+    return 0;
+}
+
+/* For a symmetric graph: number of vertices whose parent breaks the closed
+ * form given depths (parent[v] must be v's largest neighbour one level up,
+ * -1 if unreached, s for s).  O(m), parallel; used at sizes where the serial
+ * brute force is slow. */
+int64_t oracle_max_rule(int64_t n, const int64_t* off, const int32_t* adj, int32_t s,
+                        const int32_t* parent, const int32_t* depth) {
+    int64_t bad = 0;
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : bad)
+    for (int64_t v = 0; v < n; ++v) {
+        int32_t want;
+        if (v == s) want = s;
+        else if (depth[v] == n) want = -1;
+        else {
+            want = -1;
+            for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+                const int32_t u = adj[j];
+                if (depth[u] == depth[v] - 1 && u > want) want = u;
+            }
+        }
+        bad += parent[v] != want;
+    }
+    return bad;
+}

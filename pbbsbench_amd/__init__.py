@@ -1,0 +1,24 @@
+"""MI355X (gfx950) deterministic breadth-first search, a drop-in for PBBS
+breadthFirstSearch/deterministicBFS.  The work runs in libpbbs_bfs.so (HIP
+kernels + host engine) behind the C ABI of include/pbbs_bfs.h."""
+from ._lib import PbbsError, LIB_PATH  # noqa: F401
+from .core import (  # noqa: F401
+    BFS,
+    Context,
+    DeviceGraph,
+    HostGraph,
+    from_edges,
+    grid3d,
+    randlocal,
+    read_graph_from_file,
+    read_int_seq_from_file,
+    rmat,
+    write_graph_to_file,
+    write_sequence_to_file,
+)
+
+__all__ = [
+    "BFS", "Context", "DeviceGraph", "HostGraph", "PbbsError", "from_edges", "grid3d",
+    "randlocal", "read_graph_from_file", "read_int_seq_from_file", "rmat",
+    "write_graph_to_file", "write_sequence_to_file",
+]

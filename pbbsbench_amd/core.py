@@ -1,0 +1,259 @@
+"""Python face of the C ABI, shaped like the reference's benchmark interface.
+
+Reference (benchmarks/breadthFirstSearch/bench/BFS.h:4-11):
+
+    using vertexId = int; using edgeId = uint;
+    parlay::sequence<vertexId> BFS(vertexId start, const Graph& G, bool verbose);
+
+and its harness (bench/BFSTime.C:51-61) reads the graph with
+readGraphFromFile (common/graphIO.h:191-215) and writes parents with
+writeSequenceToFile (common/sequenceIO.h:189-193).  Here:
+
+    G  = read_graph_from_file(path)        # HostGraph (numpy CSR)
+    ctx = Context(0)
+    dg = ctx.upload(G)                     # DeviceGraph, resident in HBM
+    parents = BFS(start, dg)               # numpy int32[n], -1 = unvisited
+    write_sequence_to_file(parents, path)
+
+Every call goes through libpbbs_bfs.so; nothing here computes a BFS on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class HostGraph:
+    """Host CSR: int64 offsets[n+1] (offsets[n] == m), int32 adj[m]."""
+
+    def __init__(self, offsets, adj):
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.adj = np.ascontiguousarray(adj, dtype=np.int32)
+        if self.offsets.ndim != 1 or self.offsets.size < 2:
+            raise ValueError("offsets must hold n+1 >= 2 entries")
+        if int(self.offsets[-1]) != self.adj.size:
+            raise ValueError("offsets[n] must equal len(adj)")
+
+    @property
+    def n(self):
+        return self.offsets.size - 1
+
+    @property
+    def m(self):
+        return self.adj.size
+
+    def numVertices(self):  # common/graph.h:114
+        return self.n
+
+    def degrees(self):
+        return np.diff(self.offsets)
+
+    def _as_csr(self):
+        c = _lib.Csr()
+        c.n = self.n
+        c.m = self.m
+        c.offsets = self.offsets.ctypes.data_as(C.POINTER(C.c_int64))
+        c.adj = self.adj.ctypes.data_as(C.POINTER(C.c_int32))
+        return c
+
+    @staticmethod
+    def _take(csr):
+        """Copies a library-allocated pbbs_csr into numpy and frees it."""
+        try:
+            n, m = csr.n, csr.m
+            off = np.ctypeslib.as_array(csr.offsets, shape=(n + 1,)).copy()
+            adj = (np.ctypeslib.as_array(csr.adj, shape=(m,)).copy() if m > 0
+                   else np.zeros(0, dtype=np.int32))
+        finally:
+            lib().pbbs_csr_free(C.byref(csr))
+        return HostGraph(off, adj)
+
+
+# ------------------------------------------------------------ constructors
+def from_edges(n, u, v, symmetrise=True):
+    """CSR from an edge list; self-loops dropped when symmetrising, duplicates
+    removed (common/graphUtils.h:54-94)."""
+    u = np.ascontiguousarray(u, dtype=np.int32)
+    v = np.ascontiguousarray(v, dtype=np.int32)
+    if u.shape != v.shape:
+        raise ValueError("u and v differ in length")
+    c = _lib.Csr()
+    check(lib().pbbs_csr_from_edges(n, u.size, _ptr(u), _ptr(v), 1 if symmetrise else 0,
+                                    C.byref(c)), "csr_from_edges")
+    return HostGraph._take(c)
+
+
+def rmat(scale, draws, a, b, c, seed=1, permute_seed=0):
+    """Symmetric RMAT graph of 2**scale vertices (testData/graphData/rMatGraph.C:33-79)."""
+    out = _lib.Csr()
+    check(lib().pbbs_gen_rmat(scale, draws, a, b, c, seed, permute_seed, C.byref(out)),
+          "gen_rmat")
+    return HostGraph._take(out)
+
+
+def grid3d(d):
+    """d x d x d torus, degree 6 (testData/graphData/gridGraph.C:53-72)."""
+    out = _lib.Csr()
+    check(lib().pbbs_gen_grid3d(d, C.byref(out)), "gen_grid3d")
+    return HostGraph._take(out)
+
+
+def randlocal(n, k=10, seed=1):
+    """Vertex i draws k neighbours (i +- floor(n**u)) mod n, symmetrised."""
+    out = _lib.Csr()
+    check(lib().pbbs_gen_randlocal(n, k, seed, C.byref(out)), "gen_randlocal")
+    return HostGraph._take(out)
+
+
+# ------------------------------------------------------------------- text I/O
+def read_graph_from_file(path):
+    """PBBS AdjacencyGraph reader (common/graphIO.h:191-215)."""
+    out = _lib.Csr()
+    check(lib().pbbs_read_adjacency_graph(str(path).encode(), C.byref(out)),
+          "readGraphFromFile")
+    return HostGraph._take(out)
+
+
+def write_graph_to_file(g, path):
+    """PBBS AdjacencyGraph writer (common/graphIO.h:78-105)."""
+    c = g._as_csr()
+    check(lib().pbbs_write_adjacency_graph(str(path).encode(), C.byref(c)), "writeGraphToFile")
+
+
+def write_sequence_to_file(a, path):
+    """sequenceInt writer (common/sequenceIO.h:189-193, common/IO.h:151-166)."""
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    check(lib().pbbs_write_sequence_int(str(path).encode(), _ptr(a), a.size),
+          "writeSequenceToFile")
+
+
+def read_int_seq_from_file(path):
+    """sequenceInt reader (common/IO.h:217-229)."""
+    p = C.POINTER(C.c_int32)()
+    n = C.c_int64()
+    check(lib().pbbs_read_sequence_int(str(path).encode(), C.byref(p), C.byref(n)),
+          "readIntSeqFromFile")
+    try:
+        return (np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value
+                else np.zeros(0, dtype=np.int32))
+    finally:
+        lib().pbbs_free(C.cast(p, C.c_void_p))
+
+
+# ------------------------------------------------------------ device objects
+class Context:
+    """One GPU: a stream, a BFS workspace and captured level graphs."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().pbbs_ctx_create(device, C.byref(h)), "pbbs_ctx_create")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().pbbs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def tuning(self):
+        t = _lib.Tuning()
+        check(lib().pbbs_tuning_get(self._h, C.byref(t)), "tuning_get")
+        return t
+
+    def set_tuning(self, **kw):
+        t = self.tuning
+        for k, v in kw.items():
+            setattr(t, k, v)
+        check(lib().pbbs_tuning_set(self._h, C.byref(t)), "tuning_set")
+
+    def upload(self, g, wide_offsets=False, assume_symmetric=False):
+        """Copies a HostGraph into HBM.  wide_offsets keeps int64 offsets;
+        otherwise they are narrowed to uint32 (the reference's edgeId)."""
+        flags = 0
+        if wide_offsets:
+            off = g.offsets
+            flags |= _lib.PBBS_GRAPH_OFFSETS_I64
+        else:
+            if g.m >= 2**32:
+                raise ValueError("m >= 2^32 needs wide offsets")
+            off = g.offsets.astype(np.uint32)
+        if assume_symmetric:
+            flags |= _lib.PBBS_GRAPH_ASSUME_SYMMETRIC
+        h = C.c_void_p()
+        adj = g.adj if g.m > 0 else np.zeros(1, dtype=np.int32)
+        check(lib().pbbs_graph_upload(self._h, g.n, g.m, _ptr(off), _ptr(adj), flags,
+                                      C.byref(h)), "pbbs_graph_upload")
+        return DeviceGraph(self, h, g.n, g.m)
+
+
+class DeviceGraph:
+    """A CSR resident in HBM."""
+
+    def __init__(self, ctx, handle, n, m):
+        self.ctx = ctx
+        self._h = handle
+        self.n = n
+        self.m = m
+
+    def free(self):
+        if self._h:
+            lib().pbbs_graph_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def numVertices(self):
+        return self.n
+
+    @property
+    def info(self):
+        i = _lib.GraphInfo()
+        check(lib().pbbs_graph_info_get(self._h, C.byref(i)), "graph_info")
+        return {name: getattr(i, name) for name, _ in i._fields_ if name != "pad"}
+
+    def bfs(self, source, depth=True):
+        """Returns (parent, depth or None, stats dict); outputs in host memory."""
+        parent = np.empty(self.n, dtype=np.int32)
+        dep = np.empty(self.n, dtype=np.int32) if depth else None
+        st = _lib.BfsStats()
+        check(lib().pbbs_bfs(self.ctx._h, self._h, int(source), _ptr(parent),
+                             _ptr(dep) if depth else None, 0, C.byref(st)), "BFS")
+        return parent, dep, st.as_dict()
+
+    def bfs_device(self, source, parent_ptr=None, depth_ptr=None):
+        """BFS with outputs left in HBM (device pointers, or the context's own
+        buffers when None).  Returns the stats dict."""
+        st = _lib.BfsStats()
+        check(lib().pbbs_bfs(self.ctx._h, self._h, int(source), parent_ptr, depth_ptr,
+                             _lib.PBBS_OUT_DEVICE, C.byref(st)), "BFS")
+        return st.as_dict()
+
+
+def BFS(start, G, verbose=False):
+    """Parent sequence of a BFS from `start`: -1 = unvisited, the start
+    points to itself (bench/BFS.h:9-11)."""
+    parent, _, _ = G.bfs(start, depth=False)
+    return parent

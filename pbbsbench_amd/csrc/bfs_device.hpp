@@ -1,0 +1,75 @@
+// Shared between the HIP kernels (bfs_kernels.hip) and the host engine
+// (engine.cpp): the device-resident BFS state and the argument block the
+// level kernels read.
+#pragma once
+#include <stdint.h>
+
+namespace pbbs {
+
+constexpr int kWave = 64;             // CDNA wavefront width
+constexpr int kBlock = 256;           // threads per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kSubQueues = 8;         // frontier sub-queues (one append counter each)
+constexpr int kWaveBuf = 512;         // LDS staging entries per wavefront
+constexpr int kMaxGrid = 1 << 16;     // level-kernel workgroups at most
+constexpr int kLevelLogMax = 4096;    // per-level records kept for stats/tuning
+
+enum : int32_t { kTopDown = 0, kBottomUp = 1 };
+
+// One record per executed level: direction and the frontier it consumed.
+struct LevelRecord {
+  int32_t mode;
+  uint32_t count;   // |F_k| (vertices of degree > 0)
+  uint64_t edges;   // sum of degrees over F_k
+};
+
+// Device-resident control block of one BFS run.
+//
+// A frontier F_k is the concatenation of kSubQueues sub-queues; ctr[k % 3][s]
+// packs sub-queue s's vertex count (low 32 bits) and degree sum (high 32
+// bits), so one 64-bit atomicAdd reserves queue slots and edge offsets
+// together.  Degree-0 vertices are reached but not queued; d0[k % 3] counts
+// those at depth k.  Workgroup 0 of level k's kernel does the bookkeeping
+// (stats, modes[k & 3], clearing the counters level k+1 will fill); every
+// workgroup derives the level's direction from the same inputs itself.
+struct BfsState {
+  unsigned long long ctr[3][kSubQueues];
+  uint32_t d0[3];
+  int32_t modes[4];
+  int32_t finished;                  // set by the first kernel that sees F_k empty
+  int32_t td_levels;
+  int32_t bu_levels;
+  int32_t max_depth;
+  int32_t pad;
+  unsigned long long reached;        // n_r
+  unsigned long long edges_reached;  // m_r
+  LevelRecord log[kLevelLogMax];
+};
+
+// Everything a level kernel needs.  The init kernel copies it into device
+// memory so one captured HIP graph of level launches serves every run that
+// uses the same workspace.
+template <typename OffT>
+struct BfsArgs {
+  const OffT* off;             // offsets[n+1]
+  const int32_t* adj;          // adjacency[m]
+  int32_t* parent;             // out: parent[n]
+  int32_t* depth;              // out: depth[n]
+  unsigned long long* visited; // bitmap, ceil(n/64) words; pad bits set
+  unsigned long long* fb[3];   // frontier bitmaps, fb[k % 3] = F_k
+  int32_t* q[2];               // queues, q[k & 1] = F_k: kSubQueues regions of qcap
+  uint32_t* qe[2];             // per entry: exclusive degree prefix within its sub-queue
+  BfsState* st;
+  uint32_t* wg_level;          // per-workgroup level counter (kMaxGrid entries)
+  int32_t* host_status;        // pinned host memory: [0] done, [1] levels started
+  int64_t n;
+  int64_t m;
+  int64_t nwords;              // ceil(n/64)
+  int64_t qcap;                // entries per sub-queue region (= n)
+  int32_t td_to_bu_div;        // 0 = never bottom-up
+  int32_t bu_to_td_div;
+  int32_t bu_serial_depth;
+  int32_t symmetric;
+};
+
+}  // namespace pbbs

@@ -1,0 +1,741 @@
+// gfx950 kernels of the deterministic breadth-first search.
+//
+// One BFS = one init launch + one level launch per BFS level.  A level
+// kernel runs either top-down (sparse frontier queue) or bottom-up (dense
+// frontier bitmap).  Every workgroup derives the level number from its own
+// counter and the direction from the frontier counters, so no workgroup waits
+// for another and the host never waits between levels.
+//
+// Output rule (deterministicBFS/BFS.C:59-74 with common/atomics.h:75-81):
+// parent[v] = max{u in F_k : (u,v) in E} for v first reached at level k+1.
+//   * top-down: every frontier edge whose target was not reached before this
+//     level does atomicMax(&parent[v], u).  parent starts at -1, so the one
+//     atomic that returns -1 is the first claim; that lane owns the append of
+//     v to the next frontier.  The final value is the max whatever the order.
+//     The "reached before this level" guard is depth[v] > k: depth only moves
+//     n -> k+1 during level k and both values pass the guard, so a claimer
+//     never hides v from a later claimer of the same level (the property the
+//     reference gets from setting visited only in its winner pass, BFS.C:73).
+//   * bottom-up: an unreached v scans its sorted row from the top and stops at
+//     the first neighbour in F_k; that neighbour is the largest one in F_k.
+//     Unsorted rows are scanned completely with a max.
+//
+// Same-address device atomics serialise (about 10 ns each), so appends to
+// the next frontier are staged per wavefront in LDS and reserved once per
+// workgroup, spread over kSubQueues counters.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bfs_device.hpp"
+
+namespace pbbs {
+
+// ---------------------------------------------------------------- wave utils
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+// Number of set bits of `mask` in lanes below this one.
+__device__ __forceinline__ unsigned lanes_below(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const unsigned y = __shfl_up(x, d);
+    if (lane >= (unsigned)d) x += y;
+  }
+  return x;
+}
+
+__device__ __forceinline__ int wave_max(int x) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) x = max(x, __shfl_xor(x, d));
+  return x;
+}
+
+// Orders this wavefront's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t shfl_off(uint32_t x, int src) { return __shfl(x, src); }
+__device__ __forceinline__ int64_t shfl_off(int64_t x, int src) {
+  return (int64_t)__shfl((long long)x, src);
+}
+
+__device__ __forceinline__ bool in_set(const unsigned* bits, int u) {
+  return (bits[u >> 5] >> (u & 31)) & 1u;
+}
+
+__device__ __forceinline__ long long global_wave() {
+  return (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+}
+__device__ __forceinline__ long long num_waves() {
+  return (long long)gridDim.x * kWavesPerBlock;
+}
+
+// ------------------------------------------------------------- level view
+// The pointers one level uses, resolved once from the rotating buffers so
+// that no array of the argument block is indexed dynamically.
+template <typename OffT>
+struct LevelView {
+  const OffT* __restrict__ off;
+  const int32_t* __restrict__ adj;
+  int32_t* parent;
+  int32_t* depth;
+  unsigned long long* visited;
+  const unsigned* fb_cur;           // F_k
+  unsigned long long* fb_next;      // F_{k+1}
+  const int32_t* q_cur;             // sub-queue regions of F_k
+  const uint32_t* qe_cur;
+  int32_t* q_next;                  // this workgroup's sub-queue region of F_{k+1}
+  uint32_t* qe_next;
+  unsigned long long* ctr_next;     // its counter
+  uint32_t* d0_next;
+  const unsigned long long* ctr_cur;  // F_k's counters (all sub-queues)
+  long long n;
+  long long nwords;
+  long long qcap;
+  int k;
+  int serial_depth;
+};
+
+// --------------------------------------------------------- frontier staging
+// Per-wavefront LDS buffer of (vertex, degree offset within the buffer).
+struct StageBuf {
+  int32_t v[kWavesPerBlock][kWaveBuf];
+  uint32_t eo[kWavesPerBlock][kWaveBuf];
+  uint32_t cnt[kWavesPerBlock];
+  uint32_t deg[kWavesPerBlock];
+  unsigned long long base;
+};
+
+struct Stage {
+  unsigned cnt = 0;   // wave-uniform
+  unsigned deg = 0;   // wave-uniform
+};
+
+// Writes buffer entries [0, cnt) of wavefront w to the next frontier at
+// (qbase, ebase) and empties it.
+template <typename OffT>
+__device__ __forceinline__ void stage_copy_out(const LevelView<OffT>& a, StageBuf& sb, int w,
+                                               unsigned cnt, unsigned qbase, unsigned ebase) {
+  for (unsigned i = lane_id(); i < cnt; i += kWave) {
+    a.q_next[qbase + i] = sb.v[w][i];
+    a.qe_next[qbase + i] = ebase + sb.eo[w][i];
+  }
+}
+
+template <typename OffT>
+__device__ __forceinline__ void stage_flush_wave(const LevelView<OffT>& a, StageBuf& sb, Stage& s) {
+  const int w = threadIdx.x / kWave;
+  unsigned long long base = 0;
+  if (lane_id() == 0)
+    base = atomicAdd(a.ctr_next, ((unsigned long long)s.deg << 32) | s.cnt);
+  base = __shfl(base, 0);
+  wave_lds_sync();
+  stage_copy_out(a, sb, w, s.cnt, (unsigned)base, (unsigned)(base >> 32));
+  wave_lds_sync();
+  s.cnt = 0;
+  s.deg = 0;
+}
+
+// Stages the lanes with `take` (vertex v of degree deg) for F_{k+1}.
+template <typename OffT>
+__device__ __forceinline__ void stage_push(const LevelView<OffT>& a, StageBuf& sb, Stage& s,
+                                           bool take, int v, uint32_t deg) {
+  const unsigned long long mask = __ballot(take);
+  if (mask == 0) return;
+  const unsigned c = (unsigned)__popcll(mask);
+  const unsigned incl = wave_inclusive_sum(take ? deg : 0u);
+  const unsigned total = __shfl(incl, kWave - 1);
+  if (s.cnt + c > (unsigned)kWaveBuf) stage_flush_wave(a, sb, s);
+  if (take) {
+    const int w = threadIdx.x / kWave;
+    const unsigned pos = s.cnt + lanes_below(mask);
+    sb.v[w][pos] = v;
+    sb.eo[w][pos] = s.deg + incl - deg;
+  }
+  s.cnt += c;
+  s.deg += total;
+}
+
+// End of level: one reservation for the whole workgroup, then copy-out.
+template <typename OffT>
+__device__ __forceinline__ void stage_flush_block(const LevelView<OffT>& a, StageBuf& sb,
+                                                  const Stage& s) {
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    sb.cnt[w] = s.cnt;
+    sb.deg[w] = s.deg;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned tc = 0, td = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      tc += sb.cnt[i];
+      td += sb.deg[i];
+    }
+    sb.base = tc ? atomicAdd(a.ctr_next, ((unsigned long long)td << 32) | tc) : 0ull;
+  }
+  __syncthreads();
+  unsigned qb = (unsigned)sb.base, eb = (unsigned)(sb.base >> 32);
+  for (int i = 0; i < w; ++i) {
+    qb += sb.cnt[i];
+    eb += sb.deg[i];
+  }
+  stage_copy_out(a, sb, w, s.cnt, qb, eb);
+}
+
+// ------------------------------------------------------------------ top-down
+// Edges [eb, ee) of one sub-queue (nf entries, E edges): find the entry
+// holding edge eb with a 64-ary search over qe, then walk windows of 64
+// entries; lane l of a batch takes edge base+l and finds its source by a
+// register binary search over the window's qe values.
+template <typename OffT>
+__device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                         const int32_t* __restrict__ q, const uint32_t* __restrict__ qe,
+                         unsigned nf, unsigned long long E, unsigned long long eb,
+                         unsigned long long ee) {
+  const unsigned lane = lane_id();
+  const int k = a.k;
+  // Largest i with qe[i] <= eb (qe[0] == 0 and qe is strictly increasing).
+  unsigned lo = 0, hi = nf;
+  while (hi - lo > (unsigned)kWave) {
+    const unsigned step = (hi - lo + kWave - 1) / kWave;
+    const unsigned idx = lo + lane * step;
+    const bool p = idx < hi && qe[idx] <= eb;
+    const unsigned cnt = (unsigned)__popcll(__ballot(p));
+    lo += (cnt - 1) * step;
+    hi = min(hi, lo + step);
+  }
+  unsigned i;
+  {
+    const unsigned idx = lo + lane;
+    const bool p = idx < hi && qe[idx] <= eb;
+    i = lo + (unsigned)__popcll(__ballot(p)) - 1;
+  }
+
+  unsigned long long e = eb;
+  while (e < ee) {
+    const unsigned idx = i + lane;
+    int32_t qv = 0;
+    uint32_t qo = 0xffffffffu;
+    OffT rs = 0;
+    if (idx < nf) {
+      qv = q[idx];
+      qo = qe[idx];
+      rs = a.off[qv];
+    }
+    const unsigned long long wend = (i + kWave < nf) ? (unsigned long long)qe[i + kWave] : E;
+    const unsigned long long lim = wend < ee ? wend : ee;
+    for (unsigned long long base = e; base < lim; base += kWave) {
+      const unsigned long long my = base + lane;
+      int j = 0;
+#pragma unroll
+      for (int s = kWave / 2; s >= 1; s >>= 1) {
+        const uint32_t qt = __shfl(qo, j + s);
+        if ((unsigned long long)qt <= my) j += s;
+      }
+      const int u = __shfl(qv, j);
+      const OffT rsj = shfl_off(rs, j);
+      const uint32_t qoj = __shfl(qo, j);
+      bool fresh = false;
+      int v = 0;
+      if (my < lim) {
+        v = a.adj[(unsigned long long)rsj + (my - qoj)];
+        if (a.depth[v] > k) {
+          const int old = atomicMax(&a.parent[v], u);
+          if (old == -1) {
+            fresh = true;
+            a.depth[v] = k + 1;
+          }
+        }
+      }
+      if (__ballot(fresh)) {
+        uint32_t deg = 0;
+        if (fresh) {
+          deg = (uint32_t)(a.off[v + 1] - a.off[v]);
+          atomicOr(reinterpret_cast<unsigned*>(a.fb_next) + (v >> 5), 1u << (v & 31));
+        }
+        const unsigned long long z = __ballot(fresh && deg == 0);
+        if (z && lane == 0) atomicAdd(a.d0_next, (unsigned)__popcll(z));
+        stage_push(a, sb, stg, fresh && deg > 0, v, deg);
+      }
+    }
+    e = lim;
+    i += kWave;
+  }
+}
+
+// The frontier's edge space is the concatenation of the sub-queues; each
+// wavefront takes an equal slice of it.
+template <typename OffT>
+__device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                         unsigned long long E) {
+  const unsigned long long W = (unsigned long long)num_waves();
+  const unsigned long long gw = (unsigned long long)global_wave();
+  const unsigned long long eb = E * gw / W, ee = E * (gw + 1) / W;
+  if (eb >= ee) return;
+  unsigned long long P = 0;
+  for (int s = 0; s < kSubQueues; ++s) {
+    const unsigned long long c = a.ctr_cur[s];
+    const unsigned long long Es = c >> 32;
+    const unsigned long long lo = eb > P ? eb : P;
+    const unsigned long long hi = ee < P + Es ? ee : P + Es;
+    if (lo < hi)
+      td_range(a, sb, stg, a.q_cur + s * a.qcap, a.qe_cur + s * a.qcap, (unsigned)c, Es, lo - P,
+               hi - P);
+    P += Es;
+  }
+}
+
+// ----------------------------------------------------------------- bottom-up
+// A wavefront owns 64 consecutive vertices (one visited / frontier word).
+// Each unreached lane first tests its own top `serial_depth` neighbours,
+// four loads at a time; rows still unresolved are then scanned by the whole
+// wavefront, 64 entries per step from the top, stopping at the first hit.
+template <typename OffT, bool kSorted>
+__device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg) {
+  const unsigned lane = lane_id();
+  const int k = a.k;
+  const unsigned* __restrict__ fbc = a.fb_cur;
+  const unsigned long long* __restrict__ fbc64 = reinterpret_cast<const unsigned long long*>(fbc);
+  const long long W = num_waves();
+  for (long long b = global_wave(); b < a.nwords; b += W) {
+    const unsigned long long old = a.visited[b];
+    const unsigned long long vis = old | fbc64[b];  // F_k counts as reached
+    if (vis == ~0ull) {
+      if (vis != old && lane == 0) a.visited[b] = vis;
+      continue;
+    }
+    const bool mine = !((vis >> lane) & 1ull);
+    const int v = (int)(b * kWave + lane);
+    OffT lo = 0, hi = 0;
+    if (mine) {
+      lo = a.off[v];
+      hi = a.off[v + 1];
+    }
+    int found = -1;
+    if (kSorted) {
+      OffT p = hi;
+      for (int t = 0; t < a.serial_depth; t += 4) {
+        if (mine && found < 0 && p > lo) {
+          const OffT have = p - lo;
+          const int c0 = a.adj[p - 1];
+          const int c1 = have > 1 ? a.adj[p - 2] : -1;
+          const int c2 = have > 2 ? a.adj[p - 3] : -1;
+          const int c3 = have > 3 ? a.adj[p - 4] : -1;
+          const bool h0 = in_set(fbc, c0);
+          const bool h1 = c1 >= 0 && in_set(fbc, c1);
+          const bool h2 = c2 >= 0 && in_set(fbc, c2);
+          const bool h3 = c3 >= 0 && in_set(fbc, c3);
+          found = h0 ? c0 : h1 ? c1 : h2 ? c2 : h3 ? c3 : -1;
+          p = have > 4 ? p - 4 : lo;
+        }
+      }
+      unsigned long long left = __ballot(mine && found < 0 && p > lo);
+      while (left) {
+        const int L = __builtin_ctzll(left);
+        left &= left - 1;
+        const OffT loL = shfl_off(lo, L);
+        OffT pL = shfl_off(p, L);
+        int fu = -1;
+        while (pL > loL) {
+          const long long pos = (long long)pL - 1 - (long long)lane;
+          bool hit = false;
+          int u = -1;
+          if (pos >= (long long)loL) {
+            u = a.adj[pos];
+            hit = in_set(fbc, u);
+          }
+          const unsigned long long hm = __ballot(hit);
+          if (hm) {
+            fu = __shfl(u, __builtin_ctzll(hm));
+            break;
+          }
+          pL = (pL - loL > (OffT)kWave) ? pL - (OffT)kWave : loL;
+        }
+        if (lane == (unsigned)L) found = fu;
+      }
+    } else {
+      const bool serial = mine && (hi - lo) <= (OffT)kWave;
+      if (serial) {
+        for (OffT j = lo; j < hi; ++j) {
+          const int u = a.adj[j];
+          if (u > found && in_set(fbc, u)) found = u;
+        }
+      }
+      unsigned long long left = __ballot(mine && !serial);
+      while (left) {
+        const int L = __builtin_ctzll(left);
+        left &= left - 1;
+        const OffT loL = shfl_off(lo, L);
+        const OffT hiL = shfl_off(hi, L);
+        int best = -1;
+        for (OffT base = loL; base < hiL; base += (OffT)kWave) {
+          const OffT pos = base + (OffT)lane;
+          if (pos < hiL) {
+            const int u = a.adj[pos];
+            if (u > best && in_set(fbc, u)) best = u;
+          }
+        }
+        best = wave_max(best);
+        if (lane == (unsigned)L) found = best;
+      }
+    }
+    const bool nw = mine && found >= 0;
+    const unsigned long long nm = __ballot(nw);
+    if (nw) {
+      a.parent[v] = found;
+      a.depth[v] = k + 1;
+    }
+    if (lane == 0) {
+      if ((vis | nm) != old) a.visited[b] = vis | nm;
+      if (nm) a.fb_next[b] = nm;
+    }
+    stage_push(a, sb, stg, nw, v, (uint32_t)(hi - lo));
+  }
+}
+
+// -------------------------------------------------------------- level driver
+template <typename OffT>
+__device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsigned long long cnt,
+                                          unsigned long long E) {
+  if (!A.symmetric || A.td_to_bu_div <= 0) return kTopDown;
+  if (prev == kTopDown)
+    return (cnt + E > (unsigned long long)A.m / (unsigned)A.td_to_bu_div) ? kBottomUp : kTopDown;
+  return ((long long)cnt > A.n / A.bu_to_td_div) ? kBottomUp : kTopDown;
+}
+
+template <typename OffT, bool kSorted>
+__global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
+  __shared__ StageBuf sb;
+  BfsState* st = A->st;
+  const int k = (int)A->wg_level[blockIdx.x];
+  const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
+  unsigned long long cnt = 0, E = 0;
+#pragma unroll
+  for (int s = 0; s < kSubQueues; ++s) {
+    const unsigned long long c = st->ctr[c0][s];
+    cnt += (uint32_t)c;
+    E += c >> 32;
+  }
+  if (cnt == 0) {
+    // F_k has no vertex to expand: the BFS is over.  The first kernel to see
+    // it closes the books; later (overshoot) launches return at once.
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->finished) {
+      const uint32_t z = st->d0[c0];
+      st->reached += z;
+      if (z) st->max_depth = k;
+      st->finished = 1;
+      __threadfence_system();
+      __hip_atomic_store(&A->host_status[1], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&A->host_status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+  const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
+  const int mode = level_mode(*A, prev, cnt, E);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->modes[k & 3] = mode;
+    st->reached += cnt + st->d0[c0];
+    st->edges_reached += E;
+    st->max_depth = k;
+    if (k < kLevelLogMax) {
+      st->log[k].mode = mode;
+      st->log[k].count = (uint32_t)cnt;
+      st->log[k].edges = E;
+    }
+    if (mode == kTopDown) st->td_levels += 1; else st->bu_levels += 1;
+    // F_{k+2}'s counters held F_{k-1}, which no kernel reads any more.
+#pragma unroll
+    for (int s = 0; s < kSubQueues; ++s) st->ctr[c2][s] = 0;
+    st->d0[c2] = 0;
+    __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  {
+    // F_{k+2}'s bitmap held F_{k-1}.  A top-down level also folds F_k into
+    // the visited bitmap, which only bottom-up levels read.
+    unsigned long long* z = A->fb[c2];
+    const unsigned long long* f = A->fb[c0];
+    unsigned long long* vis = A->visited;
+    const bool fold = mode == kTopDown && A->symmetric && A->td_to_bu_div > 0;
+    const long long nwords = A->nwords;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += stride) {
+      z[w] = 0;
+      if (fold) {
+        const unsigned long long fw = f[w];
+        if (fw) vis[w] |= fw;
+      }
+    }
+  }
+  const int sq = blockIdx.x % kSubQueues;
+  LevelView<OffT> a;
+  a.off = A->off;
+  a.adj = A->adj;
+  a.parent = A->parent;
+  a.depth = A->depth;
+  a.visited = A->visited;
+  a.fb_cur = reinterpret_cast<const unsigned*>(A->fb[c0]);
+  a.fb_next = A->fb[c1];
+  a.qcap = A->qcap;
+  a.q_cur = A->q[k & 1];
+  a.qe_cur = A->qe[k & 1];
+  a.q_next = A->q[(k + 1) & 1] + sq * a.qcap;
+  a.qe_next = A->qe[(k + 1) & 1] + sq * a.qcap;
+  a.ctr_next = &st->ctr[c1][sq];
+  a.d0_next = &st->d0[c1];
+  a.ctr_cur = st->ctr[c0];
+  a.n = A->n;
+  a.nwords = A->nwords;
+  a.k = k;
+  a.serial_depth = A->bu_serial_depth;
+  Stage stg;
+  if (mode == kTopDown)
+    top_down(a, sb, stg, E);
+  else
+    bottom_up<OffT, kSorted>(a, sb, stg);
+  stage_flush_block(a, sb, stg);
+  if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsArgs<OffT>* dst,
+                                                          const unsigned long long* __restrict__ iso,
+                                                          int32_t s) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nth = (long long)gridDim.x * blockDim.x;
+  const int32_t n = (int32_t)a.n;
+  for (long long i = tid; i < a.n; i += nth) {
+    a.parent[i] = (i == s) ? s : -1;
+    a.depth[i] = (i == s) ? 0 : n;
+  }
+  const long long sw = s >> 6;
+  const unsigned long long sb = 1ull << (s & 63);
+  const unsigned long long pad = (a.n & 63) ? (~0ull << (a.n & 63)) : 0ull;
+  for (long long w = tid; w < a.nwords; w += nth) {
+    unsigned long long vv = iso[w];
+    if (w == a.nwords - 1) vv |= pad;
+    if (w == sw) vv |= sb;
+    a.visited[w] = vv;
+    a.fb[0][w] = (w == sw) ? sb : 0ull;
+    a.fb[1][w] = 0ull;
+    a.fb[2][w] = 0ull;
+  }
+  for (long long i = tid; i < kMaxGrid; i += nth) a.wg_level[i] = 0;
+  if (tid == 0) {
+    *dst = a;
+    const unsigned long long deg = (unsigned long long)(a.off[s + 1] - a.off[s]);
+    BfsState* st = a.st;
+    a.q[0][0] = s;
+    a.qe[0][0] = 0;
+    for (int j = 0; j < 3; ++j) {
+      for (int q = 0; q < kSubQueues; ++q) st->ctr[j][q] = 0;
+      st->d0[j] = 0;
+    }
+    if (deg) st->ctr[0][0] = (deg << 32) | 1ull;
+    else st->d0[0] = 1;
+    for (int j = 0; j < 4; ++j) st->modes[j] = kTopDown;
+    st->finished = 0;
+    st->td_levels = 0;
+    st->bu_levels = 0;
+    st->max_depth = 0;
+    st->reached = 0;
+    st->edges_reached = 0;
+    __threadfence_system();
+    __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// ---------------------------------------------------------- graph metadata
+// counters: [0] isolated vertices, [1] max degree, [2] bad offsets (decreasing
+// or beyond m), [3] row-start descents (adj[off[v]-1] > adj[off[v]] for a
+// non-empty row v), [4] all descents (adj[j] > adj[j+1]), [5] out-of-range
+// ids, [6] edges whose reverse is missing.
+//
+// Pass 1 reads offsets only, so malformed offsets are caught before any
+// adjacency access depends on them.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void graph_offsets_kernel(
+    const OffT* __restrict__ off, long long n, long long m, long long nwords,
+    unsigned long long* __restrict__ iso, unsigned long long* __restrict__ counters) {
+  const unsigned lane = lane_id();
+  for (long long b = global_wave(); b < nwords; b += num_waves()) {
+    const long long v = b * kWave + lane;
+    bool isolated = false, bad = false;
+    unsigned long long deg = 0;
+    if (v < n) {
+      const OffT o0 = off[v], o1 = off[v + 1];
+      bad = o1 < o0 || (long long)o1 > m;
+      if (!bad) {
+        deg = (unsigned long long)(o1 - o0);
+        isolated = deg == 0;
+      }
+    }
+    const unsigned long long im = __ballot(isolated);
+    const unsigned long long bm = __ballot(bad);
+    unsigned long long md = deg;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const unsigned long long o = __shfl_xor(md, d);
+      md = o > md ? o : md;
+    }
+    if (lane == 0) {
+      iso[b] = im;
+      if (im) atomicAdd(&counters[0], (unsigned long long)__popcll(im));
+      if (md) atomicMax(&counters[1], md);
+      if (bm) atomicAdd(&counters[2], (unsigned long long)__popcll(bm));
+    }
+  }
+}
+
+// Pass 2 (offsets validated: 0 = off[0] <= ... <= off[n] = m).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void graph_row_start_kernel(
+    const OffT* __restrict__ off, const int32_t* __restrict__ adj, long long n,
+    unsigned long long* __restrict__ counters) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v0 = (long long)blockIdx.x * blockDim.x; v0 < n; v0 += stride) {
+    const long long v = v0 + threadIdx.x;
+    bool descent = false;
+    if (v < n) {
+      const OffT o0 = off[v], o1 = off[v + 1];
+      if (o1 > o0 && o0 > 0) descent = adj[o0 - 1] > adj[o0];
+    }
+    const unsigned long long dm = __ballot(descent);
+    if (lane_id() == 0 && dm) atomicAdd(&counters[3], (unsigned long long)__popcll(dm));
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void graph_edge_meta_kernel(
+    const int32_t* __restrict__ adj, long long m, long long n,
+    unsigned long long* __restrict__ counters) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long j0 = (long long)blockIdx.x * blockDim.x; j0 < m; j0 += stride) {
+    const long long j = j0 + threadIdx.x;
+    bool descent = false, out_of_range = false;
+    if (j < m) {
+      const int32_t x = adj[j];
+      out_of_range = x < 0 || (long long)x >= n;
+      if (j + 1 < m) descent = x > adj[j + 1];
+    }
+    const unsigned long long dm = __ballot(descent);
+    const unsigned long long om = __ballot(out_of_range);
+    if (lane_id() == 0) {
+      if (dm) atomicAdd(&counters[4], (unsigned long long)__popcll(dm));
+      if (om) atomicAdd(&counters[5], (unsigned long long)__popcll(om));
+    }
+  }
+}
+
+// Needs validated offsets, in-range ids and sorted rows.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void graph_symmetry_kernel(
+    const OffT* __restrict__ off, const int32_t* __restrict__ adj, long long n, long long m,
+    unsigned long long* __restrict__ counters) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long j0 = (long long)blockIdx.x * blockDim.x; j0 < m; j0 += stride) {
+    const long long j = j0 + threadIdx.x;
+    bool missing = false;
+    if (j < m) {
+      // source row u: largest u with off[u] <= j
+      long long lo = 0, hi = n;  // off[lo] <= j < off[hi]
+      while (hi - lo > 1) {
+        const long long mid = (lo + hi) >> 1;
+        if ((long long)off[mid] <= j) lo = mid; else hi = mid;
+      }
+      const int32_t u = (int32_t)lo;
+      const int32_t v = adj[j];
+      long long a0 = (long long)off[v], a1 = (long long)off[v + 1];
+      while (a0 < a1) {
+        const long long mid = (a0 + a1) >> 1;
+        if (adj[mid] < u) a0 = mid + 1; else a1 = mid;
+      }
+      missing = !(a0 < (long long)off[v + 1] && adj[a0] == u);
+    }
+    const unsigned long long mm = __ballot(missing);
+    if (lane_id() == 0 && mm) atomicAdd(&counters[6], (unsigned long long)__popcll(mm));
+  }
+}
+
+// ------------------------------------------------------------ host launchers
+template <typename OffT>
+hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
+                           const unsigned long long* iso, int32_t s, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(bfs_init_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, a, dst, iso, s);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+const void* level_kernel_symbol(bool sorted) {
+  return sorted ? reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true>)
+                : reinterpret_cast<const void*>(&bfs_level_kernel<OffT, false>);
+}
+
+template <typename OffT>
+hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, bool sorted, int grid, hipStream_t stream) {
+  if (sorted)
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+  else
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, false>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_graph_offsets(const OffT* off, long long n, long long m,
+                                unsigned long long* iso, unsigned long long* counters, int grid,
+                                hipStream_t stream) {
+  const long long nwords = (n + kWave - 1) / kWave;
+  hipLaunchKernelGGL(graph_offsets_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, n, m,
+                     nwords, iso, counters);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_graph_rows(const OffT* off, const int32_t* adj, long long n, long long m,
+                             unsigned long long* counters, int grid, hipStream_t stream) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(graph_row_start_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, adj,
+                     n, counters);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(graph_edge_meta_kernel, dim3(grid), dim3(kBlock), 0, stream, adj, m, n,
+                     counters);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long n, long long m,
+                                 unsigned long long* counters, int grid, hipStream_t stream) {
+  if (m == 0) return hipSuccess;
+  hipLaunchKernelGGL(graph_symmetry_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, adj,
+                     n, m, counters);
+  return hipGetLastError();
+}
+
+#define PBBS_INSTANTIATE(OffT)                                                                   \
+  template hipError_t launch_bfs_init<OffT>(const BfsArgs<OffT>&, BfsArgs<OffT>*,                \
+                                            const unsigned long long*, int32_t, int,             \
+                                            hipStream_t);                                        \
+  template const void* level_kernel_symbol<OffT>(bool);                                         \
+  template hipError_t launch_bfs_level<OffT>(const BfsArgs<OffT>*, bool, int, hipStream_t);      \
+  template hipError_t launch_graph_offsets<OffT>(const OffT*, long long, long long,             \
+                                                 unsigned long long*, unsigned long long*, int,  \
+                                                 hipStream_t);                                   \
+  template hipError_t launch_graph_rows<OffT>(const OffT*, const int32_t*, long long, long long, \
+                                              unsigned long long*, int, hipStream_t);            \
+  template hipError_t launch_graph_symmetry<OffT>(const OffT*, const int32_t*, long long,        \
+                                                  long long, unsigned long long*, int,           \
+                                                  hipStream_t);
+
+PBBS_INSTANTIATE(uint32_t)
+PBBS_INSTANTIATE(int64_t)
+
+}  // namespace pbbs

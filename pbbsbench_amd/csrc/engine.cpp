@@ -1,0 +1,519 @@
+// Host side of the C ABI: contexts, graphs resident in HBM, the per-context
+// BFS workspace, and the level loop (init launch, then captured HIP graphs of
+// level launches until the device reports an empty frontier).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "../../include/pbbs_bfs.h"
+#include "bfs_device.hpp"
+#include "engine.hpp"
+
+namespace pbbs {
+
+template <typename OffT>
+hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
+                           const unsigned long long* iso, int32_t s, int grid, hipStream_t stream);
+template <typename OffT>
+const void* level_kernel_symbol(bool sorted);
+template <typename OffT>
+hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, bool sorted, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_graph_offsets(const OffT* off, long long n, long long m,
+                                unsigned long long* iso, unsigned long long* counters, int grid,
+                                hipStream_t stream);
+template <typename OffT>
+hipError_t launch_graph_rows(const OffT* off, const int32_t* adj, long long n, long long m,
+                             unsigned long long* counters, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long n, long long m,
+                                 unsigned long long* counters, int grid, hipStream_t stream);
+
+// ------------------------------------------------------------------- errors
+static thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+void clear_error() { g_last_error.clear(); }
+const char* last_error() { return g_last_error.c_str(); }
+
+}  // namespace pbbs
+
+using namespace pbbs;
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(PBBS_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+// ------------------------------------------------------------------ context
+struct pbbs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  pbbs_tuning tuning{};
+  // workspace, sized for the largest graph seen so far
+  int64_t cap_n = 0;
+  int32_t* w_parent = nullptr;
+  int32_t* w_depth = nullptr;
+  unsigned long long* w_bits = nullptr;  // visited, fb0, fb1, fb2
+  int32_t* w_q = nullptr;                // q0 | q1, each kSubQueues regions of cap_n
+  uint32_t* w_qe = nullptr;              // qe0 | qe1, same layout
+  uint32_t* w_wg_level = nullptr;        // kMaxGrid per-workgroup level counters
+  BfsState* w_state = nullptr;
+  void* w_args = nullptr;                // device copy of BfsArgs<OffT>
+  int32_t* host_status = nullptr;        // pinned: [0] done, [1] levels finished
+  BfsState* host_state = nullptr;        // pinned: stats read-back
+  hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> level_graphs;
+};
+
+struct pbbs_graph {
+  pbbs_ctx* ctx = nullptr;
+  int64_t n = 0, m = 0;
+  bool off64 = false;
+  bool sorted = false;
+  bool symmetric = false;
+  void* d_off = nullptr;
+  int32_t* d_adj = nullptr;
+  unsigned long long* d_iso = nullptr;  // degree-0 bitmap
+  int64_t isolated = 0;
+  int64_t max_degree = 0;
+  int64_t device_bytes = 0;
+};
+
+static void free_workspace(pbbs_ctx* c) {
+  (void)hipFree(c->w_parent);
+  (void)hipFree(c->w_depth);
+  (void)hipFree(c->w_bits);
+  (void)hipFree(c->w_q);
+  (void)hipFree(c->w_qe);
+  c->w_parent = c->w_depth = nullptr;
+  c->w_bits = nullptr;
+  c->w_q = nullptr;
+  c->w_qe = nullptr;
+  c->cap_n = 0;
+}
+
+static int ensure_workspace(pbbs_ctx* c, int64_t n) {
+  if (n <= c->cap_n) return PBBS_OK;
+  free_workspace(c);
+  const int64_t nwords = (n + kWave - 1) / kWave;
+  if (hipMalloc(&c->w_parent, n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->w_depth, n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->w_bits, 4 * nwords * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->w_q, 2 * kSubQueues * n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->w_qe, 2 * kSubQueues * n * sizeof(uint32_t)) != hipSuccess) {
+    free_workspace(c);
+    return fail(PBBS_ERR_NOMEM, "workspace allocation for n=%lld failed", (long long)n);
+  }
+  c->cap_n = n;
+  return PBBS_OK;
+}
+
+static int default_tuning(pbbs_tuning* t) {
+  std::memset(t, 0, sizeof *t);
+  t->td_to_bu_div = 20;
+  t->bu_to_td_div = 20;
+  t->grid_blocks = 0;
+  t->bu_serial_depth = 4;
+  t->chunk_levels = 16;
+  t->use_graphs = 1;
+  return PBBS_OK;
+}
+
+extern "C" {
+
+const char* pbbs_last_error(void) { return last_error(); }
+const char* pbbs_version(void) { return "pbbs-bfs-gfx950 0.1"; }
+
+int pbbs_ctx_create(int device, pbbs_ctx** out) {
+  clear_error();
+  if (!out) return fail(PBBS_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  HIP_TRY(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count)
+    return fail(PBBS_ERR_INVALID, "device %d out of range (%d devices)", device, count);
+  HIP_TRY(hipSetDevice(device));
+  pbbs_ctx* c = new pbbs_ctx();
+  c->device = device;
+  default_tuning(&c->tuning);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->w_state, sizeof(BfsState)) != hipSuccess ||
+      hipMalloc(&c->w_wg_level, kMaxGrid * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->w_args, sizeof(BfsArgs<int64_t>)) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_status, 64, hipHostMallocCoherent) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_state, sizeof(BfsState), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
+      hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess) {
+    pbbs_ctx_destroy(c);
+    return fail(PBBS_ERR_HIP, "context setup on device %d failed", device);
+  }
+  c->num_cus = prop.multiProcessorCount;
+  std::memset(c->host_status, 0, 64);
+  *out = c;
+  return PBBS_OK;
+}
+
+int pbbs_ctx_destroy(pbbs_ctx* c) {
+  if (!c) return PBBS_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& kv : c->level_graphs) (void)hipGraphExecDestroy(kv.second);
+  free_workspace(c);
+  (void)hipFree(c->w_state);
+  (void)hipFree(c->w_wg_level);
+  (void)hipFree(c->w_args);
+  if (c->host_status) (void)hipHostFree(c->host_status);
+  if (c->host_state) (void)hipHostFree(c->host_state);
+  if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+  if (c->ev_end) (void)hipEventDestroy(c->ev_end);
+  if (c->ev_c0) (void)hipEventDestroy(c->ev_c0);
+  if (c->ev_c1) (void)hipEventDestroy(c->ev_c1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return PBBS_OK;
+}
+
+int pbbs_tuning_get(const pbbs_ctx* c, pbbs_tuning* out) {
+  clear_error();
+  if (!c || !out) return fail(PBBS_ERR_INVALID, "NULL argument");
+  *out = c->tuning;
+  return PBBS_OK;
+}
+
+int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
+  clear_error();
+  if (!c || !t) return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
+      t->bu_serial_depth < 0 || t->bu_serial_depth > 64 || t->chunk_levels < 1 ||
+      t->chunk_levels > 4096)
+    return fail(PBBS_ERR_INVALID,
+                "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
+                "0<=bu_serial_depth<=64, 1<=chunk_levels<=4096)");
+  c->tuning = *t;
+  return PBBS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------- graphs
+template <typename OffT>
+static int upload_checks(pbbs_ctx* c, pbbs_graph* g, uint32_t flags) {
+  hipStream_t s = c->stream;
+  const int64_t nwords = (g->n + kWave - 1) / kWave;
+  unsigned long long* d_cnt = nullptr;
+  HIP_TRY(hipMalloc(&d_cnt, 8 * sizeof(unsigned long long)));
+  unsigned long long h[8];
+  int rc = PBBS_OK;
+  const int grid = std::max(1, c->num_cus * 8);
+  const OffT* off = static_cast<const OffT*>(g->d_off);
+  OffT ends[2];
+  do {
+    if (hipMemsetAsync(d_cnt, 0, 8 * sizeof(unsigned long long), s) != hipSuccess ||
+        launch_graph_offsets<OffT>(off, g->n, g->m, g->d_iso, d_cnt, grid, s) != hipSuccess ||
+        hipMemcpyAsync(h, d_cnt, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&ends[0], off, sizeof(OffT), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&ends[1], off + g->n, sizeof(OffT), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(PBBS_ERR_HIP, "graph offset check failed: %s", hipGetErrorString(hipGetLastError()));
+      break;
+    }
+    if ((int64_t)ends[0] != 0 || (int64_t)ends[1] != g->m || h[2] != 0) {
+      rc = fail(PBBS_ERR_FORMAT,
+                "bad offsets: offsets[0]=%lld offsets[n]=%lld (m=%lld), %llu decreasing or >m",
+                (long long)ends[0], (long long)ends[1], (long long)g->m, h[2]);
+      break;
+    }
+    if (launch_graph_rows<OffT>(off, g->d_adj, g->n, g->m, d_cnt, grid, s) != hipSuccess ||
+        hipMemcpyAsync(h, d_cnt, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(PBBS_ERR_HIP, "graph row check failed");
+      break;
+    }
+    if (h[5] != 0) {
+      rc = fail(PBBS_ERR_FORMAT, "%llu adjacency entries outside [0, n)", h[5]);
+      break;
+    }
+    g->isolated = (int64_t)h[0];
+    g->max_degree = (int64_t)h[1];
+    g->sorted = h[4] == h[3];
+    if (flags & PBBS_GRAPH_ASSUME_SYMMETRIC) {
+      g->symmetric = true;
+    } else if (g->sorted) {
+      if (launch_graph_symmetry<OffT>(off, g->d_adj, g->n, g->m, d_cnt, grid, s) != hipSuccess ||
+          hipMemcpyAsync(h, d_cnt, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(PBBS_ERR_HIP, "graph symmetry check failed");
+        break;
+      }
+      g->symmetric = h[6] == 0;
+    } else {
+      g->symmetric = false;
+    }
+  } while (false);
+  (void)nwords;
+  (void)hipFree(d_cnt);
+  return rc;
+}
+
+extern "C" {
+
+int pbbs_graph_free(pbbs_graph* g) {
+  if (!g) return PBBS_OK;
+  if (g->ctx) {
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipStreamSynchronize(g->ctx->stream);
+  }
+  (void)hipFree(g->d_off);
+  (void)hipFree(g->d_adj);
+  (void)hipFree(g->d_iso);
+  delete g;
+  return PBBS_OK;
+}
+
+int pbbs_graph_upload(pbbs_ctx* c, int64_t n, int64_t m, const void* offsets, const int32_t* adj,
+                      uint32_t flags, pbbs_graph** out) {
+  clear_error();
+  if (!c || !out || !offsets) return fail(PBBS_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (n <= 0 || n > INT32_MAX) return fail(PBBS_ERR_LIMIT, "n=%lld outside [1, 2^31)", (long long)n);
+  if (m < 0 || m > (int64_t)UINT32_MAX)
+    return fail(PBBS_ERR_LIMIT, "m=%lld outside [0, 2^32)", (long long)m);
+  if (m > 0 && !adj) return fail(PBBS_ERR_INVALID, "adj is NULL with m > 0");
+  HIP_TRY(hipSetDevice(c->device));
+  pbbs_graph* g = new pbbs_graph();
+  g->ctx = c;
+  g->n = n;
+  g->m = m;
+  g->off64 = (flags & PBBS_GRAPH_OFFSETS_I64) != 0;
+  const size_t osz = g->off64 ? sizeof(int64_t) : sizeof(uint32_t);
+  const int64_t nwords = (n + kWave - 1) / kWave;
+  const hipMemcpyKind kind =
+      (flags & PBBS_GRAPH_DEVICE_INPUT) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  if (hipMalloc(&g->d_off, (n + 1) * osz) != hipSuccess ||
+      hipMalloc(&g->d_adj, std::max<int64_t>(m, 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&g->d_iso, nwords * sizeof(unsigned long long)) != hipSuccess) {
+    pbbs_graph_free(g);
+    return fail(PBBS_ERR_NOMEM, "graph allocation (n=%lld, m=%lld) failed", (long long)n,
+                (long long)m);
+  }
+  g->device_bytes = (n + 1) * osz + std::max<int64_t>(m, 1) * 4 + nwords * 8;
+  if (hipMemcpyAsync(g->d_off, offsets, (n + 1) * osz, kind, c->stream) != hipSuccess ||
+      (m > 0 && hipMemcpyAsync(g->d_adj, adj, m * sizeof(int32_t), kind, c->stream) != hipSuccess) ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
+    pbbs_graph_free(g);
+    return fail(PBBS_ERR_HIP, "graph copy into HBM failed");
+  }
+  const int rc = g->off64 ? upload_checks<int64_t>(c, g, flags) : upload_checks<uint32_t>(c, g, flags);
+  if (rc != PBBS_OK) {
+    pbbs_graph_free(g);
+    return rc;
+  }
+  *out = g;
+  return PBBS_OK;
+}
+
+int pbbs_graph_info_get(const pbbs_graph* g, pbbs_graph_info* out) {
+  clear_error();
+  if (!g || !out) return fail(PBBS_ERR_INVALID, "NULL argument");
+  std::memset(out, 0, sizeof *out);
+  out->n = g->n;
+  out->m = g->m;
+  out->offsets_i64 = g->off64;
+  out->sorted = g->sorted;
+  out->symmetric = g->symmetric;
+  out->isolated = g->isolated;
+  out->max_degree = g->max_degree;
+  out->device_bytes = g->device_bytes;
+  return PBBS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------- BFS
+static int level_grid(pbbs_ctx* c, bool off64, bool sorted) {
+  if (c->tuning.grid_blocks > 0) return std::min(c->tuning.grid_blocks, kMaxGrid);
+  int per_cu = 0;
+  const void* fn = off64 ? level_kernel_symbol<int64_t>(sorted) : level_kernel_symbol<uint32_t>(sorted);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return std::min(per_cu * c->num_cus, kMaxGrid);
+}
+
+template <typename OffT>
+static int get_level_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphExec_t* out) {
+  const auto key = std::make_tuple((int)sizeof(OffT), (int)sorted, grid, K);
+  auto it = c->level_graphs.find(key);
+  if (it != c->level_graphs.end()) {
+    *out = it->second;
+    return PBBS_OK;
+  }
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int i = 0; i < K && le == hipSuccess; ++i)
+    le = launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), sorted, grid,
+                                c->stream);
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(PBBS_ERR_HIP, "capturing %d level launches failed: %s", K,
+                hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(PBBS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+  c->level_graphs[key] = exec;
+  *out = exec;
+  return PBBS_OK;
+}
+
+static int next_chunk(int base, int launched) {
+  int k = base;
+  while (k < launched && k < 1024) k *= 2;
+  return k;
+}
+
+template <typename OffT>
+static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
+                   pbbs_bfs_stats* stats) {
+  const int64_t nwords = (g->n + kWave - 1) / kWave;
+  BfsArgs<OffT> a;
+  std::memset(&a, 0, sizeof a);
+  a.off = static_cast<const OffT*>(g->d_off);
+  a.adj = g->d_adj;
+  a.parent = d_parent;
+  a.depth = d_depth;
+  a.visited = c->w_bits;
+  a.fb[0] = c->w_bits + nwords;
+  a.fb[1] = c->w_bits + 2 * nwords;
+  a.fb[2] = c->w_bits + 3 * nwords;
+  a.q[0] = c->w_q;
+  a.q[1] = c->w_q + kSubQueues * c->cap_n;
+  a.qe[0] = c->w_qe;
+  a.qe[1] = c->w_qe + kSubQueues * c->cap_n;
+  a.qcap = c->cap_n;
+  a.st = c->w_state;
+  a.wg_level = c->w_wg_level;
+  a.host_status = c->host_status;
+  a.n = g->n;
+  a.m = g->m;
+  a.nwords = nwords;
+  a.td_to_bu_div = c->tuning.td_to_bu_div;
+  a.bu_to_td_div = c->tuning.bu_to_td_div;
+  a.bu_serial_depth = c->tuning.bu_serial_depth;
+  a.symmetric = g->symmetric ? 1 : 0;
+
+  const int grid = level_grid(c, g->off64, g->sorted);
+  const int init_grid =
+      (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
+  volatile int32_t* hs = c->host_status;
+  hs[0] = 0;
+  hs[1] = 0;
+  int K = c->tuning.chunk_levels;
+  hipGraphExec_t exec = nullptr;
+  // capture (first use only) while the stream is idle
+  if (c->tuning.use_graphs) {
+    const int rc = get_level_graph<OffT>(c, g->sorted, grid, K, &exec);
+    if (rc != PBBS_OK) return rc;
+  }
+  HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  HIP_TRY(launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(c->w_args), g->d_iso, s, init_grid,
+                                c->stream));
+  int64_t launched = 0;
+  for (;;) {
+    if (c->tuning.use_graphs) {
+      if (!exec) {
+        const int rc = get_level_graph<OffT>(c, g->sorted, grid, K, &exec);
+        if (rc != PBBS_OK) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(exec, c->stream));
+      exec = nullptr;
+    } else {
+      for (int i = 0; i < K; ++i)
+        HIP_TRY(launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), g->sorted,
+                                       grid, c->stream));
+    }
+    launched += K;
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (hs[0]) break;
+    if (launched > g->n + 2)
+      return fail(PBBS_ERR_STATE, "BFS did not finish after %lld level launches", (long long)launched);
+    K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(launched, 1 << 20));
+  }
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
+  if (stats) {
+    HIP_TRY(hipMemcpyAsync(c->host_state, c->w_state, offsetof(BfsState, log),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memset(stats, 0, sizeof *stats);
+    stats->levels = c->host_state->max_depth + 1;
+    stats->td_levels = c->host_state->td_levels;
+    stats->bu_levels = c->host_state->bu_levels;
+    stats->launches = (int32_t)launched;
+    stats->reached = (int64_t)c->host_state->reached;
+    stats->edges_reached = (int64_t)c->host_state->edges_reached;
+    stats->device_ms = ms;
+  }
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* parent,
+                        int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats) {
+  clear_error();
+  if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
+  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (source < 0 || (int64_t)source >= g->n)
+    return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = ensure_workspace(c, g->n);
+  if (rc != PBBS_OK) return rc;
+  const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
+  int32_t* dp = (dev_out && parent) ? parent : c->w_parent;
+  int32_t* dd = (dev_out && depth) ? depth : c->w_depth;
+  rc = g->off64 ? run_bfs<int64_t>(c, g, source, dp, dd, stats)
+                : run_bfs<uint32_t>(c, g, source, dp, dd, stats);
+  if (rc != PBBS_OK) return rc;
+  if (!dev_out && (parent || depth)) {
+    HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
+    if (parent)
+      HIP_TRY(hipMemcpyAsync(parent, dp, g->n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    if (depth)
+      HIP_TRY(hipMemcpyAsync(depth, dd, g->n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_c1));
+    if (stats) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1));
+      stats->copy_ms = ms;
+    }
+  }
+  return PBBS_OK;
+}
