@@ -72,7 +72,8 @@ typedef struct {
     int32_t td_to_bu_div;   /* default 20; 0 disables bottom-up entirely */
     int32_t bu_to_td_div;   /* default 20 */
     int32_t grid_blocks;    /* workgroups per level launch; 0 = auto (occupancy x CUs) */
-    int32_t bu_serial_depth;/* neighbours a lane tests alone before its wavefront helps; default 4 */
+    int32_t bu_serial_depth;/* row entries a lane tests alone (bottom-up) before its
+                               wavefront takes over the row; default 32 */
     int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16 */
     int32_t use_graphs;     /* 1 = replay captured HIP graphs of level launches (default),
                                0 = launch each level kernel on the stream */
@@ -134,6 +135,23 @@ typedef struct {
  * BFS(start, G, verbose) of bench/BFS.h:11 / deterministicBFS/BFS.C:39. */
 int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
              int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats);
+
+/* Per-level timing of one BFS, for tuning and measurement: every level kernel
+ * is launched on its own with a hipEvent after it (no graph replay), so the
+ * times exclude launch gaps.  Writes up to `max_levels` records and the number
+ * of levels in *count; *init_ms gets the init kernel's time.  Outputs stay in
+ * the context's device buffers.  No reference counterpart (the reference's
+ * per-round timers print wall time only, common/ligraLight.h:115-126). */
+typedef struct {
+    int32_t level;          /* k: this kernel consumed F_k */
+    int32_t mode;           /* 0 top-down, 1 bottom-up */
+    int64_t frontier;       /* |F_k| (vertices of degree > 0) */
+    int64_t frontier_edges; /* sum of degrees over F_k */
+    double  ms;             /* kernel time */
+} pbbs_level_profile;
+
+int pbbs_bfs_profile(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, pbbs_level_profile* out,
+                     int32_t max_levels, int32_t* count, double* init_ms);
 
 /* ---- host-side CSR container, generators and PBBS text I/O ------------- */
 /* A host CSR: int64 offsets[n+1], int32 adj[m], rows ascending, no duplicate

@@ -66,6 +66,16 @@ class BfsStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class LevelProfile(C.Structure):
+    _fields_ = [
+        ("level", C.c_int32),
+        ("mode", C.c_int32),
+        ("frontier", C.c_int64),
+        ("frontier_edges", C.c_int64),
+        ("ms", C.c_double),
+    ]
+
+
 class Csr(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -90,6 +100,9 @@ SIGNATURES = [
     ("pbbs_graph_info_get", C.c_int, [_P, C.POINTER(GraphInfo)]),
     ("pbbs_graph_free", C.c_int, [_P]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
+    ("pbbs_bfs_profile", C.c_int,
+     [_P, _P, C.c_int32, C.POINTER(LevelProfile), C.c_int32, C.POINTER(C.c_int32),
+      C.POINTER(C.c_double)]),
     ("pbbs_csr_free", None, [C.POINTER(Csr)]),
     ("pbbs_gen_rmat", C.c_int,
      [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,

@@ -243,6 +243,20 @@ class DeviceGraph:
                              _ptr(dep) if depth else None, 0, C.byref(st)), "BFS")
         return parent, dep, st.as_dict()
 
+    def profile(self, source, max_levels=4096):
+        """Per-level kernel times of one BFS (levels launched one by one).
+        Returns (init_ms, [ {level, mode, frontier, frontier_edges, ms}, ... ])."""
+        recs = (_lib.LevelProfile * max_levels)()
+        cnt = C.c_int32()
+        init = C.c_double()
+        check(lib().pbbs_bfs_profile(self.ctx._h, self._h, int(source), recs, max_levels,
+                                     C.byref(cnt), C.byref(init)), "bfs_profile")
+        out = []
+        for r in recs[:min(cnt.value, max_levels)]:
+            out.append({"level": r.level, "mode": "BU" if r.mode else "TD",
+                        "frontier": r.frontier, "frontier_edges": r.frontier_edges, "ms": r.ms})
+        return init.value, out
+
     def bfs_device(self, source, parent_ptr=None, depth_ptr=None):
         """BFS with outputs left in HBM (device pointers, or the context's own
         buffers when None).  Returns the stats dict."""

@@ -30,6 +30,22 @@
 
 namespace pbbs {
 
+#ifdef PBBS_WAVE_TIMING
+// Diagnostic build only: per-wavefront wall-clock stamps (start, end of the
+// first pass, end) of the first 16 levels, for 16384 wavefronts.
+constexpr int kTimingLevels = 16;
+constexpr int kTimingWaves = 16384;
+__device__ unsigned long long g_wave_times[kTimingLevels * kTimingWaves * 3];
+__device__ __forceinline__ void stamp(int k, int slot) {
+  const long long gw = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (k < kTimingLevels && gw < kTimingWaves && __lane_id() == 0)
+    g_wave_times[((long long)k * kTimingWaves + gw) * 3 + slot] = wall_clock64();
+}
+#define PBBS_STAMP(k, slot) stamp((k), (slot))
+#else
+#define PBBS_STAMP(k, slot) ((void)0)
+#endif
+
 // ---------------------------------------------------------------- wave utils
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
@@ -192,15 +208,14 @@ __device__ __forceinline__ void stage_flush_block(const LevelView<OffT>& a, Stag
 }
 
 // ------------------------------------------------------------------ top-down
-// Edges [eb, ee) of one sub-queue (nf entries, E edges): find the entry
-// holding edge eb with a 64-ary search over qe, then walk windows of 64
-// entries; lane l of a batch takes edge base+l and finds its source by a
-// register binary search over the window's qe values.
+// Edges [eb, ee) of one sub-queue (nf entries, E edges; E < 2^32 since m is):
+// find the entry holding edge eb with a 64-ary search over qe, then walk
+// windows of 64 entries; lane l of a batch takes edge base+l and finds its
+// source by a register binary search over the window's qe values.
 template <typename OffT>
 __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                          const int32_t* __restrict__ q, const uint32_t* __restrict__ qe,
-                         unsigned nf, unsigned long long E, unsigned long long eb,
-                         unsigned long long ee) {
+                         unsigned nf, uint32_t E, uint32_t eb, uint32_t ee) {
   const unsigned lane = lane_id();
   const int k = a.k;
   // Largest i with qe[i] <= eb (qe[0] == 0 and qe is strictly increasing).
@@ -220,7 +235,7 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     i = lo + (unsigned)__popcll(__ballot(p)) - 1;
   }
 
-  unsigned long long e = eb;
+  uint32_t e = eb;
   while (e < ee) {
     const unsigned idx = i + lane;
     int32_t qv = 0;
@@ -231,23 +246,24 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
       qo = qe[idx];
       rs = a.off[qv];
     }
-    const unsigned long long wend = (i + kWave < nf) ? (unsigned long long)qe[i + kWave] : E;
-    const unsigned long long lim = wend < ee ? wend : ee;
-    for (unsigned long long base = e; base < lim; base += kWave) {
-      const unsigned long long my = base + lane;
+    const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : E;
+    const uint32_t lim = min(wend, ee);
+    for (uint32_t base = e; base < lim; base += kWave) {
+      const uint32_t my = base + lane;  // < lim + 64 <= 2^32 - 1 + 64: compare in 64 bits
+      const bool valid = (unsigned long long)base + lane < (unsigned long long)lim;
       int j = 0;
 #pragma unroll
       for (int s = kWave / 2; s >= 1; s >>= 1) {
         const uint32_t qt = __shfl(qo, j + s);
-        if ((unsigned long long)qt <= my) j += s;
+        if (qt <= my) j += s;
       }
       const int u = __shfl(qv, j);
       const OffT rsj = shfl_off(rs, j);
       const uint32_t qoj = __shfl(qo, j);
       bool fresh = false;
       int v = 0;
-      if (my < lim) {
-        v = a.adj[(unsigned long long)rsj + (my - qoj)];
+      if (valid) {
+        v = a.adj[rsj + (OffT)(my - qoj)];
         if (a.depth[v] > k) {
           const int old = atomicMax(&a.parent[v], u);
           if (old == -1) {
@@ -288,118 +304,266 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     const unsigned long long lo = eb > P ? eb : P;
     const unsigned long long hi = ee < P + Es ? ee : P + Es;
     if (lo < hi)
-      td_range(a, sb, stg, a.q_cur + s * a.qcap, a.qe_cur + s * a.qcap, (unsigned)c, Es, lo - P,
-               hi - P);
+      td_range(a, sb, stg, a.q_cur + s * a.qcap, a.qe_cur + s * a.qcap, (unsigned)c,
+               (uint32_t)Es, (uint32_t)(lo - P), (uint32_t)(hi - P));
     P += Es;
   }
 }
 
 // ----------------------------------------------------------------- bottom-up
-// A wavefront owns 64 consecutive vertices (one visited / frontier word).
-// Each unreached lane first tests its own top `serial_depth` neighbours,
-// four loads at a time; rows still unresolved are then scanned by the whole
-// wavefront, 64 entries per step from the top, stopping at the first hit.
-template <typename OffT, bool kSorted>
-__device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg) {
-  const unsigned lane = lane_id();
-  const int k = a.k;
+// Each wavefront owns a contiguous run of frontier words (64 vertices each).
+// It folds F_k into the visited bitmap word by word and gathers the ids of
+// the still-unreached vertices into an LDS queue (ascending), so that every
+// probe pass runs 64 unreached vertices at once whatever the density.  A
+// pass tests the top four entries of each row with independent loads; rows
+// ascend, so the first hit from the top is the largest neighbour in F_k.
+// Vertices left unresolved are parked and finished 64 at a time: up to
+// `serial_depth` entries per lane, then a whole-wavefront scan of the rest of
+// the row, 64 entries per step from the top.  Found vertices only set bits of
+// F_{k+1} (merged per word before one atomicOr); the visited bitmap learns
+// them when level k+1 folds F_{k+1} in.
+
+// Tests entries [p-N, p) of row [lo, p) from the top with N independent
+// loads; returns the first hit (or -1) and moves p down past them.
+template <int N, typename OffT>
+__device__ __forceinline__ int scan_top(const LevelView<OffT>& a, OffT lo, OffT& p) {
   const unsigned* __restrict__ fbc = a.fb_cur;
-  const unsigned long long* __restrict__ fbc64 = reinterpret_cast<const unsigned long long*>(fbc);
-  const long long W = num_waves();
-  for (long long b = global_wave(); b < a.nwords; b += W) {
-    const unsigned long long old = a.visited[b];
-    const unsigned long long vis = old | fbc64[b];  // F_k counts as reached
-    if (vis == ~0ull) {
-      if (vis != old && lane == 0) a.visited[b] = vis;
-      continue;
+  const OffT have = p - lo;
+  int c[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
+  bool h[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) h[j] = c[j] >= 0 && in_set(fbc, c[j]);
+  int found = -1;
+#pragma unroll
+  for (int j = N - 1; j >= 0; --j)
+    if (h[j]) found = c[j];
+  p = have > (OffT)N ? p - (OffT)N : lo;
+  return found;
+}
+
+// Whole-wavefront scan of the rows of the lanes in `left`, from the top, 64
+// entries per step; the first hit (lowest lane) is the highest entry.
+template <typename OffT>
+__device__ __forceinline__ void scan_rows_together(const LevelView<OffT>& a,
+                                                   unsigned long long left, OffT lo, OffT p,
+                                                   int& found) {
+  const unsigned lane = lane_id();
+  const unsigned* __restrict__ fbc = a.fb_cur;
+  while (left) {
+    const int L = __builtin_ctzll(left);
+    left &= left - 1;
+    const OffT loL = shfl_off(lo, L);
+    OffT pL = shfl_off(p, L);
+    int fu = -1;
+    while (pL > loL) {
+      const long long pos = (long long)pL - 1 - (long long)lane;
+      bool hit = false;
+      int u = -1;
+      if (pos >= (long long)loL) {
+        u = a.adj[pos];
+        hit = in_set(fbc, u);
+      }
+      const unsigned long long hm = __ballot(hit);
+      if (hm) {
+        fu = __shfl(u, __builtin_ctzll(hm));
+        break;
+      }
+      pL = (pL - loL > (OffT)kWave) ? pL - (OffT)kWave : loL;
     }
-    const bool mine = !((vis >> lane) & 1ull);
-    const int v = (int)(b * kWave + lane);
-    OffT lo = 0, hi = 0;
+    if (lane == (unsigned)L) found = fu;
+  }
+}
+
+// Full-row max scan for unsorted rows (no early exit).
+template <typename OffT>
+__device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine, OffT lo, OffT hi) {
+  const unsigned lane = lane_id();
+  const unsigned* __restrict__ fbc = a.fb_cur;
+  int found = -1;
+  const bool serial = mine && (hi - lo) <= (OffT)kWave;
+  if (serial) {
+    for (OffT j = lo; j < hi; ++j) {
+      const int u = a.adj[j];
+      if (u > found && in_set(fbc, u)) found = u;
+    }
+  }
+  unsigned long long left = __ballot(mine && !serial);
+  while (left) {
+    const int L = __builtin_ctzll(left);
+    left &= left - 1;
+    const OffT loL = shfl_off(lo, L);
+    const OffT hiL = shfl_off(hi, L);
+    int best = -1;
+    for (OffT base = loL; base < hiL; base += (OffT)kWave) {
+      const OffT pos = base + (OffT)lane;
+      if (pos < hiL) {
+        const int u = a.adj[pos];
+        if (u > best && in_set(fbc, u)) best = u;
+      }
+    }
+    best = wave_max(best);
+    if (lane == (unsigned)L) found = best;
+  }
+  return found;
+}
+
+template <typename OffT>
+struct BuLds {
+  int32_t q[kWavesPerBlock][2 * kWave];      // unreached vertices, ascending
+  int32_t lv[kWavesPerBlock][2 * kWave];     // parked vertices
+  OffT lp[kWavesPerBlock][2 * kWave];        // their scan positions
+};
+
+// Records the found lanes: parent/depth, their F_{k+1} bits (one atomicOr per
+// distinct word; lanes hold ascending ids so a word's lanes are adjacent),
+// and the frontier append.
+template <typename OffT>
+__device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                          bool valid, bool nw, int v, int found, uint32_t deg) {
+  const unsigned lane = lane_id();
+  if (nw) {
+    a.parent[v] = found;
+    a.depth[v] = a.k + 1;
+  }
+  const unsigned long long fm = __ballot(nw);
+  if (fm) {
+    const int wi = valid ? (v >> 6) : -1 - (int)lane;  // idle lanes join no word
+    unsigned long long bits = nw ? (1ull << (v & 63)) : 0ull;
+    // OR of `bits` over the lanes at or above this one with the same word
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const unsigned long long o = __shfl_down(bits, d);
+      const int ow = __shfl_down(wi, d);
+      if (lane + d < (unsigned)kWave && ow == wi) bits |= o;
+    }
+    const int pw = __shfl_up(wi, 1);
+    const bool head = lane == 0 || pw != wi;
+    if (head && bits) atomicOr(&a.fb_next[wi], bits);
+  }
+  stage_push(a, sb, stg, nw, v, deg);
+}
+
+template <typename OffT>
+__device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                 BuLds<OffT>& L, unsigned& npark) {
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  wave_lds_sync();
+  for (unsigned base = 0; base < npark; base += kWave) {
+    const bool mine = base + lane < npark;
+    int v = 0;
+    OffT lo = 0, hi = 0, p = 0;
     if (mine) {
+      v = L.lv[w][base + lane];
+      p = L.lp[w][base + lane];
       lo = a.off[v];
       hi = a.off[v + 1];
     }
     int found = -1;
-    if (kSorted) {
-      OffT p = hi;
-      for (int t = 0; t < a.serial_depth; t += 4) {
-        if (mine && found < 0 && p > lo) {
-          const OffT have = p - lo;
-          const int c0 = a.adj[p - 1];
-          const int c1 = have > 1 ? a.adj[p - 2] : -1;
-          const int c2 = have > 2 ? a.adj[p - 3] : -1;
-          const int c3 = have > 3 ? a.adj[p - 4] : -1;
-          const bool h0 = in_set(fbc, c0);
-          const bool h1 = c1 >= 0 && in_set(fbc, c1);
-          const bool h2 = c2 >= 0 && in_set(fbc, c2);
-          const bool h3 = c3 >= 0 && in_set(fbc, c3);
-          found = h0 ? c0 : h1 ? c1 : h2 ? c2 : h3 ? c3 : -1;
-          p = have > 4 ? p - 4 : lo;
-        }
-      }
-      unsigned long long left = __ballot(mine && found < 0 && p > lo);
-      while (left) {
-        const int L = __builtin_ctzll(left);
-        left &= left - 1;
-        const OffT loL = shfl_off(lo, L);
-        OffT pL = shfl_off(p, L);
-        int fu = -1;
-        while (pL > loL) {
-          const long long pos = (long long)pL - 1 - (long long)lane;
-          bool hit = false;
-          int u = -1;
-          if (pos >= (long long)loL) {
-            u = a.adj[pos];
-            hit = in_set(fbc, u);
-          }
-          const unsigned long long hm = __ballot(hit);
-          if (hm) {
-            fu = __shfl(u, __builtin_ctzll(hm));
-            break;
-          }
-          pL = (pL - loL > (OffT)kWave) ? pL - (OffT)kWave : loL;
-        }
-        if (lane == (unsigned)L) found = fu;
-      }
-    } else {
-      const bool serial = mine && (hi - lo) <= (OffT)kWave;
-      if (serial) {
-        for (OffT j = lo; j < hi; ++j) {
-          const int u = a.adj[j];
-          if (u > found && in_set(fbc, u)) found = u;
-        }
-      }
-      unsigned long long left = __ballot(mine && !serial);
-      while (left) {
-        const int L = __builtin_ctzll(left);
-        left &= left - 1;
-        const OffT loL = shfl_off(lo, L);
-        const OffT hiL = shfl_off(hi, L);
-        int best = -1;
-        for (OffT base = loL; base < hiL; base += (OffT)kWave) {
-          const OffT pos = base + (OffT)lane;
-          if (pos < hiL) {
-            const int u = a.adj[pos];
-            if (u > best && in_set(fbc, u)) best = u;
-          }
-        }
-        best = wave_max(best);
-        if (lane == (unsigned)L) found = best;
-      }
+    // parked rows are the deep ones: sixteen loads in flight per pass
+    for (int t = 4; t < a.serial_depth; t += 16) {
+      const bool active = mine && found < 0 && p > lo;
+      if (!__ballot(active)) break;
+      if (active) found = scan_top<16>(a, lo, p);
     }
-    const bool nw = mine && found >= 0;
-    const unsigned long long nm = __ballot(nw);
-    if (nw) {
-      a.parent[v] = found;
-      a.depth[v] = k + 1;
-    }
-    if (lane == 0) {
-      if ((vis | nm) != old) a.visited[b] = vis | nm;
-      if (nm) a.fb_next[b] = nm;
-    }
-    stage_push(a, sb, stg, nw, v, (uint32_t)(hi - lo));
+    scan_rows_together(a, __ballot(mine && found < 0 && p > lo), lo, p, found);
+    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
   }
+  wave_lds_sync();
+  npark = 0;
+}
+
+// One probe pass over queue entries [0, cnt) (cnt <= 64).
+template <typename OffT, bool kSorted>
+__device__ void bu_batch(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
+                         unsigned cnt, unsigned& npark) {
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  wave_lds_sync();
+  const bool mine = lane < cnt;
+  int v = 0;
+  OffT lo = 0, hi = 0;
+  if (mine) {
+    v = L.q[w][lane];
+    lo = a.off[v];
+    hi = a.off[v + 1];
+  }
+  int found = -1;
+  if (kSorted) {
+    OffT p = hi;
+    if (mine && p > lo) found = scan_top<4>(a, lo, p);
+    const bool park = mine && found < 0 && p > lo;
+    const unsigned long long pm = __ballot(park);
+    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
+    if (pm) {
+      if (npark + (unsigned)__popcll(pm) > 2 * kWave) bu_finish_parked(a, sb, stg, L, npark);
+      if (park) {
+        const unsigned slot = npark + lanes_below(pm);
+        L.lv[w][slot] = v;
+        L.lp[w][slot] = p;
+      }
+      npark += (unsigned)__popcll(pm);
+    }
+  } else {
+    found = scan_unsorted(a, mine, lo, hi);
+    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
+  }
+}
+
+template <typename OffT, bool kSorted>
+__device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L) {
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const unsigned long long* __restrict__ fbc64 =
+      reinterpret_cast<const unsigned long long*>(a.fb_cur);
+  const long long W = num_waves();
+  const long long gw = global_wave();
+  const long long b0 = a.nwords * gw / W, b1 = a.nwords * (gw + 1) / W;
+  unsigned nq = 0, npark = 0;
+  for (long long g0 = b0; g0 < b1; g0 += kWave) {
+    const long long mb = g0 + lane;
+    unsigned long long unv = 0;
+    if (mb < b1) {
+      const unsigned long long old = a.visited[mb];
+      const unsigned long long vis = old | fbc64[mb];  // fold F_k in
+      if (vis != old) a.visited[mb] = vis;
+      unv = ~vis;
+    }
+    unsigned long long todo = __ballot(unv != 0);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const unsigned long long word = __shfl(unv, j);
+      const bool take = (word >> lane) & 1ull;
+      const unsigned c = (unsigned)__popcll(word);
+      if (nq + c > 2 * kWave) {
+        // keep the queue below two batches: run the full batch first
+        bu_batch<OffT, kSorted>(a, sb, stg, L, kWave, npark);
+        wave_lds_sync();
+        const unsigned rest = nq - kWave;
+        int tmp = lane < rest ? L.q[w][kWave + lane] : 0;
+        wave_lds_sync();
+        if (lane < rest) L.q[w][lane] = tmp;
+        nq = rest;
+      }
+      if (take) L.q[w][nq + lanes_below(word)] = (int)((g0 + j) * kWave + lane);
+      nq += c;
+      if (nq >= (unsigned)kWave) {
+        bu_batch<OffT, kSorted>(a, sb, stg, L, kWave, npark);
+        wave_lds_sync();
+        const unsigned rest = nq - kWave;
+        int tmp = lane < rest ? L.q[w][kWave + lane] : 0;
+        wave_lds_sync();
+        if (lane < rest) L.q[w][lane] = tmp;
+        nq = rest;
+      }
+    }
+  }
+  if (nq) bu_batch<OffT, kSorted>(a, sb, stg, L, nq, npark);
+  PBBS_STAMP(a.k, 1);
+  if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark);
 }
 
 // -------------------------------------------------------------- level driver
@@ -412,9 +576,13 @@ __device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsi
   return ((long long)cnt > A.n / A.bu_to_td_div) ? kBottomUp : kTopDown;
 }
 
+#ifndef PBBS_LEVEL_MIN_BLOCKS
+#define PBBS_LEVEL_MIN_BLOCKS 1
+#endif
 template <typename OffT, bool kSorted>
-__global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
+__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
+  __shared__ BuLds<OffT> bl;
   BfsState* st = A->st;
   const int k = (int)A->wg_level[blockIdx.x];
   const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
@@ -439,6 +607,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* 
     }
     return;
   }
+  PBBS_STAMP(k, 0);
   const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
   const int mode = level_mode(*A, prev, cnt, E);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -500,7 +669,8 @@ __global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* 
   if (mode == kTopDown)
     top_down(a, sb, stg, E);
   else
-    bottom_up<OffT, kSorted>(a, sb, stg);
+    bottom_up<OffT, kSorted>(a, sb, stg, bl);
+  PBBS_STAMP(k, 2);
   stage_flush_block(a, sb, stg);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
 }
@@ -719,6 +889,17 @@ hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long 
                      n, m, counters);
   return hipGetLastError();
 }
+
+#ifdef PBBS_WAVE_TIMING
+extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
+  const long long cap = (long long)kTimingLevels * kTimingWaves * 3;
+  if (count > cap) count = cap;
+  return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_wave_times), count * 8, 0, hipMemcpyDeviceToHost) ==
+                 hipSuccess
+             ? 0
+             : -2;
+}
+#endif
 
 #define PBBS_INSTANTIATE(OffT)                                                                   \
   template hipError_t launch_bfs_init<OffT>(const BfsArgs<OffT>&, BfsArgs<OffT>*,                \

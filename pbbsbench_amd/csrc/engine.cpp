@@ -11,6 +11,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "../../include/pbbs_bfs.h"
 #include "bfs_device.hpp"
@@ -134,7 +135,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->td_to_bu_div = 20;
   t->bu_to_td_div = 20;
   t->grid_blocks = 0;
-  t->bu_serial_depth = 4;
+  t->bu_serial_depth = 32;
   t->chunk_levels = 16;
   t->use_graphs = 1;
   return PBBS_OK;
@@ -207,11 +208,11 @@ int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
   clear_error();
   if (!c || !t) return fail(PBBS_ERR_INVALID, "NULL argument");
   if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
-      t->bu_serial_depth < 0 || t->bu_serial_depth > 64 || t->chunk_levels < 1 ||
+      t->bu_serial_depth < 0 || t->bu_serial_depth > 4096 || t->chunk_levels < 1 ||
       t->chunk_levels > 4096)
     return fail(PBBS_ERR_INVALID,
                 "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
-                "0<=bu_serial_depth<=64, 1<=chunk_levels<=4096)");
+                "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096)");
   c->tuning = *t;
   return PBBS_OK;
 }
@@ -400,8 +401,7 @@ static int next_chunk(int base, int launched) {
 }
 
 template <typename OffT>
-static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
-                   pbbs_bfs_stats* stats) {
+static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth) {
   const int64_t nwords = (g->n + kWave - 1) / kWave;
   BfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
@@ -428,7 +428,13 @@ static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int
   a.bu_to_td_div = c->tuning.bu_to_td_div;
   a.bu_serial_depth = c->tuning.bu_serial_depth;
   a.symmetric = g->symmetric ? 1 : 0;
+  return a;
+}
 
+template <typename OffT>
+static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
+                   pbbs_bfs_stats* stats) {
+  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth);
   const int grid = level_grid(c, g->off64, g->sorted);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
@@ -516,4 +522,95 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
     }
   }
   return PBBS_OK;
+}
+
+// ------------------------------------------------------------ level profile
+template <typename OffT>
+static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
+                       int32_t max_levels, int32_t* count, double* init_ms) {
+  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth);
+  const int grid = level_grid(c, g->off64, g->sorted);
+  const int init_grid =
+      (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
+  volatile int32_t* hs = c->host_status;
+  hs[0] = 0;
+  hs[1] = 0;
+  std::vector<hipEvent_t> ev;
+  auto cleanup = [&]() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  };
+  auto new_event = [&]() -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev.push_back(e);
+    return e;
+  };
+  hipEvent_t e0 = new_event(), e1 = new_event();
+  if (!e0 || !e1) {
+    cleanup();
+    return fail(PBBS_ERR_HIP, "event creation failed");
+  }
+  int rc = PBBS_OK;
+  std::vector<float> ms;
+  do {
+    if (hipEventRecord(e0, c->stream) != hipSuccess ||
+        launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(c->w_args), g->d_iso, s, init_grid,
+                              c->stream) != hipSuccess ||
+        hipEventRecord(e1, c->stream) != hipSuccess) {
+      rc = fail(PBBS_ERR_HIP, "init launch failed");
+      break;
+    }
+    hipEvent_t prev = e1;
+    for (int64_t k = 0; k <= g->n + 1; ++k) {
+      hipEvent_t e = new_event();
+      if (!e || launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), g->sorted,
+                                       grid, c->stream) != hipSuccess ||
+          hipEventRecord(e, c->stream) != hipSuccess || hipEventSynchronize(e) != hipSuccess) {
+        rc = fail(PBBS_ERR_HIP, "level launch failed");
+        break;
+      }
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, prev, e);
+      ms.push_back(t);
+      prev = e;
+      if (hs[0]) break;
+    }
+  } while (false);
+  if (rc == PBBS_OK) {
+    float ti = 0.f;
+    (void)hipEventElapsedTime(&ti, e0, e1);
+    if (init_ms) *init_ms = ti;
+    if (hipMemcpy(c->host_state, c->w_state, sizeof(BfsState), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(PBBS_ERR_HIP, "state read-back failed");
+  }
+  cleanup();
+  if (rc != PBBS_OK) return rc;
+  // the last launch saw an empty frontier; the ones before it are the levels
+  const int32_t levels = (int32_t)ms.size() - 1;
+  const int32_t nrec = std::min<int32_t>(std::min<int32_t>(levels, max_levels), kLevelLogMax);
+  for (int32_t k = 0; k < nrec; ++k) {
+    out[k].level = k;
+    out[k].mode = c->host_state->log[k].mode;
+    out[k].frontier = c->host_state->log[k].count;
+    out[k].frontier_edges = (int64_t)c->host_state->log[k].edges;
+    out[k].ms = ms[k];
+  }
+  *count = levels;
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
+                                pbbs_level_profile* out, int32_t max_levels, int32_t* count,
+                                double* init_ms) {
+  clear_error();
+  if (!c || !g || !count || (max_levels > 0 && !out))
+    return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (source < 0 || (int64_t)source >= g->n)
+    return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
+  HIP_TRY(hipSetDevice(c->device));
+  const int rc = ensure_workspace(c, g->n);
+  if (rc != PBBS_OK) return rc;
+  return g->off64 ? run_profile<int64_t>(c, g, source, out, max_levels, count, init_ms)
+                  : run_profile<uint32_t>(c, g, source, out, max_levels, count, init_ms);
 }

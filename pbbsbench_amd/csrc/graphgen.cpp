@@ -37,13 +37,29 @@ int bits_for(int64_t n) {
   return std::max(b, 1);
 }
 
+// Uninitialised 64-bit key buffer: the parallel loops that fill it are the
+// first to touch its pages, so page faults spread over all threads.
+struct Keys {
+  uint64_t* p = nullptr;
+  int64_t n = 0;
+  explicit Keys(int64_t count) : p(static_cast<uint64_t*>(std::malloc(std::max<int64_t>(count, 1) * 8))), n(count) {
+    if (!p) throw std::bad_alloc();
+  }
+  Keys(const Keys&) = delete;
+  Keys& operator=(const Keys&) = delete;
+  ~Keys() { std::free(p); }
+  uint64_t& operator[](int64_t i) { return p[i]; }
+  const uint64_t& operator[](int64_t i) const { return p[i]; }
+  int64_t size() const { return n; }
+};
+
 // Stable parallel LSD radix sort of keys on their low `key_bits` bits.
-void radix_sort(std::vector<uint64_t>& a, int key_bits) {
-  const int64_t N = (int64_t)a.size();
+void radix_sort(Keys& a, int key_bits) {
+  const int64_t N = a.size();
   if (N < 2) return;
   constexpr int D = 11;
   constexpr int B = 1 << D;
-  std::vector<uint64_t> tmp(N);
+  Keys tmp(N);
   const int T = std::max(1, omp_get_max_threads());
   std::vector<int64_t> hist((size_t)T * B);
   for (int shift = 0; shift < key_bits; shift += D) {
@@ -69,13 +85,13 @@ void radix_sort(std::vector<uint64_t>& a, int key_bits) {
       int64_t* h = &hist[(size_t)t * B];
       for (int64_t i = lo; i < hi; ++i) tmp[h[(a[i] >> shift) & (B - 1)]++] = a[i];
     }
-    a.swap(tmp);
+    std::swap(a.p, tmp.p);
   }
 }
 
 // keys[i] = (u << vb) | v, sorted and deduplicated into a CSR.  Keys equal to
 // UINT64_MAX are dropped (they mark removed self-loops).
-int csr_from_keys(int64_t n, std::vector<uint64_t>& keys, int vb, pbbs_csr* out) {
+int csr_from_keys(int64_t n, Keys& keys, int vb, pbbs_csr* out) {
   radix_sort(keys, 2 * vb);
   int64_t N = (int64_t)keys.size();
   while (N > 0 && keys[N - 1] == ~0ull) --N;
@@ -130,7 +146,7 @@ int csr_from_keys(int64_t n, std::vector<uint64_t>& keys, int vb, pbbs_csr* out)
 // Relabels both endpoints of every key with a seeded random permutation
 // (Fisher-Yates), as graphReorder does before writing an unordered graph
 // (common/graphUtils.h:166-189).
-void permute_keys(std::vector<uint64_t>& keys, int64_t n, int vb, uint64_t seed) {
+void permute_keys(Keys& keys, int64_t n, int vb, uint64_t seed) {
   std::vector<int32_t> perm(n);
   for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
   uint64_t s = mix64(seed ^ 0x5DEECE66Dull);
@@ -141,7 +157,7 @@ void permute_keys(std::vector<uint64_t>& keys, int64_t n, int vb, uint64_t seed)
   }
   const uint64_t vmask = (uint64_t(1) << vb) - 1;
 #pragma omp parallel for schedule(static)
-  for (int64_t i = 0; i < (int64_t)keys.size(); ++i) {
+  for (int64_t i = 0; i < keys.size(); ++i) {
     const uint64_t k = keys[i];
     if (k == ~0ull) continue;
     keys[i] = ((uint64_t)perm[k >> vb] << vb) | (uint64_t)perm[k & vmask];
@@ -173,7 +189,7 @@ int pbbs_csr_from_edges(int64_t n, int64_t num_edges, const int32_t* u, const in
                   v[i], (long long)n);
   const int vb = bits_for(n);
   try {
-    std::vector<uint64_t> keys((size_t)num_edges * (symmetrise ? 2 : 1));
+    Keys keys(num_edges * (symmetrise ? 2 : 1));
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < num_edges; ++i) {
       const uint64_t a = (uint64_t)u[i], b = (uint64_t)v[i];
@@ -204,7 +220,7 @@ int pbbs_gen_rmat(int32_t scale, int64_t draws, double a, double b, double c, ui
   const double ab = a + b, abc = a + b + c;
   const uint64_t hseed = mix64(seed);
   try {
-    std::vector<uint64_t> keys((size_t)draws * 2);
+    Keys keys(draws * 2);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < draws; ++i) {
       uint64_t st = mix64(hseed ^ ((uint64_t)i * 0xD1B54A32D192ED03ull));
@@ -240,7 +256,7 @@ int pbbs_gen_grid3d(int32_t d, pbbs_csr* out) {
   };
   try {
     // three forward torus edges per vertex (gridGraph.C:62-70), symmetrised
-    std::vector<uint64_t> keys((size_t)n * 6);
+    Keys keys(n * 6);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < D; ++i)
       for (int64_t j = 0; j < D; ++j)
@@ -269,7 +285,7 @@ int pbbs_gen_randlocal(int64_t n, int32_t k, uint64_t seed, pbbs_csr* out) {
   const uint64_t hseed = mix64(seed ^ 0xA0761D6478BD642Full);
   const double dn = (double)n;
   try {
-    std::vector<uint64_t> keys((size_t)n * k * 2);
+    Keys keys(n * k * 2);
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
       uint64_t st = mix64(hseed ^ ((uint64_t)i * 0xD1B54A32D192ED03ull));

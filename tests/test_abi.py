@@ -1,0 +1,68 @@
+"""The C ABI library loads and exports every function include/*.h declares.
+No compute call is made here (this suite runs without a GPU)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT, has_gpu
+
+HEADERS = [os.path.join(ROOT, "include", f) for f in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if f.endswith(".h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(pbbs_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("pbbs_ctx_create", "pbbs_graph_upload", "pbbs_bfs", "pbbs_read_adjacency_graph"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from pbbsbench_amd import _lib
+    lib = _lib.lib()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert set(declared_functions()) <= bound, set(declared_functions()) - bound
+
+
+def test_version_and_error_strings():
+    from pbbsbench_amd import _lib
+    lib = _lib.lib()
+    assert lib.pbbs_version().startswith(b"pbbs-bfs-gfx950")
+    assert isinstance(lib.pbbs_last_error(), bytes)
+
+
+@pytest.mark.skipif(has_gpu(), reason="checks the no-GPU error path")
+def test_context_without_gpu_fails_cleanly():
+    import pbbsbench_amd as pb
+    with pytest.raises(pb.PbbsError):
+        pb.Context(0)
+
+
+def test_null_arguments_rejected():
+    from pbbsbench_amd import _lib
+    lib = _lib.lib()
+    assert lib.pbbs_ctx_create(0, None) == _lib.PBBS_ERR_INVALID
+    assert lib.pbbs_graph_upload(None, 1, 0, None, None, 0, None) == _lib.PBBS_ERR_INVALID
+    assert lib.pbbs_bfs(None, None, 0, None, None, 0, None) == _lib.PBBS_ERR_INVALID
+    assert b"NULL" in lib.pbbs_last_error()
+
+
+def test_driver_binary_usage():
+    exe = os.path.join(ROOT, "pbbsbench_amd", "bin", "BFS")
+    assert os.path.exists(exe)
+    out = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
+    assert "usage:" in out.stdout and "-src" in out.stdout

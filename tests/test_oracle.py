@@ -1,0 +1,126 @@
+"""Pins the oracle before anything is compared against it.
+
+The reference has no golden vectors for BFS; the pins are (a) the committed
+known-answer fixtures (tests/golden, derived by formula per graph family),
+(b) an independent brute force of the max-parent closed form, (c) the torus
+closed form, (d) planted mistakes that the checker and the max rule must
+reject.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import pbbsbench_amd as pb
+from conftest import GOLDEN, golden_cases
+
+
+def load_golden(name):
+    g = pb.read_graph_from_file(f"{GOLDEN}/{name}.adj")
+    return (g, pb.read_int_seq_from_file(f"{GOLDEN}/{name}.parent"),
+            pb.read_int_seq_from_file(f"{GOLDEN}/{name}.depth"))
+
+
+@pytest.mark.parametrize("name,meta", golden_cases(), ids=[c[0] for c in golden_cases()])
+def test_oracle_matches_golden(name, meta):
+    g, wp, wd = load_golden(name)
+    s = meta["source"]
+    p, rounds = oracle.det_bfs(g.offsets, g.adj, s)
+    assert np.array_equal(p, wp)
+    ps, ds = oracle.serial_bfs(g.offsets, g.adj, s)
+    assert np.array_equal(ps, wp) and np.array_equal(ds, wd)
+    assert np.array_equal(oracle.level_number(p), wd)
+    # rounds of deterministicBFS = levels reached by an expandable frontier
+    assert rounds >= 1
+    if meta["symmetric"]:
+        assert oracle.check_bfs(g.offsets, g.adj, s, p) == -1
+        assert oracle.max_rule_violations(g.offsets, g.adj, s, p, wd) == 0
+
+
+def random_graph(seed, n, m, symmetric=True):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, m)
+    # a mix of local and long edges, plus a few duplicates and self-loops
+    v = np.where(rng.random(m) < 0.5, (u + rng.integers(-3, 4, m)) % n, rng.integers(0, n, m))
+    return pb.from_edges(n, u, v, symmetrise=symmetric)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_det_bfs_equals_brute_force(seed):
+    n = [50, 300, 1000, 4000][seed % 4]
+    g = random_graph(seed, n, 3 * n)
+    rng = np.random.default_rng(100 + seed)
+    for s in rng.integers(0, n, 3):
+        p, _ = oracle.det_bfs(g.offsets, g.adj, int(s), threads=4)
+        ps, ds = oracle.serial_bfs(g.offsets, g.adj, int(s))
+        assert np.array_equal(p, ps)
+        assert np.array_equal(oracle.level_number(p), ds)
+        assert oracle.check_bfs(g.offsets, g.adj, int(s), p) == -1
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_det_bfs_directed_equals_brute_force(seed):
+    g = random_graph(seed, 800, 2400, symmetric=False)
+    p, _ = oracle.det_bfs(g.offsets, g.adj, 7)
+    ps, ds = oracle.serial_bfs(g.offsets, g.adj, 7)
+    assert np.array_equal(p, ps)
+    assert np.array_equal(oracle.level_number(p), ds)
+
+
+def test_det_bfs_thread_count_invariance():
+    g = pb.rmat(13, 16 * 2 ** 13, 0.57, 0.19, 0.19, seed=3)
+    ref, _ = oracle.det_bfs(g.offsets, g.adj, 1, threads=1)
+    for t in (2, 3, 8):
+        p, _ = oracle.det_bfs(g.offsets, g.adj, 1, threads=t)
+        assert np.array_equal(p, ref)
+
+
+@pytest.mark.parametrize("d", [3, 4, 7, 12])
+def test_torus_closed_form_equals_brute_force(d):
+    g = pb.grid3d(d)
+    cp, cd = oracle.torus_closed_form(d)
+    ps, ds = oracle.serial_bfs(g.offsets, g.adj, 0)
+    assert np.array_equal(cp, ps) and np.array_equal(cd, ds)
+
+
+def test_rmat_config1_oracle_consistency():
+    # RMAT 2^16, 2^19 draws, (.5,.1,.1): the CPU reference configuration.
+    g = pb.rmat(16, 2 ** 19, 0.5, 0.1, 0.1, seed=1)
+    p, _ = oracle.det_bfs(g.offsets, g.adj, 0)
+    ps, ds = oracle.serial_bfs(g.offsets, g.adj, 0)
+    assert np.array_equal(p, ps)
+    assert oracle.check_bfs(g.offsets, g.adj, 0, p) == -1
+    assert oracle.max_rule_violations(g.offsets, g.adj, 0, p, ds) == 0
+
+
+# ------------------------------------------------------- planted mistakes
+def test_checker_rejects_planted_mistakes():
+    g, wp, wd = load_golden("bintree31")
+    assert oracle.check_bfs(g.offsets, g.adj, 0, wp) == -1
+    bad = wp.copy(); bad[0] = 1                   # root does not point to itself
+    assert oracle.check_bfs(g.offsets, g.adj, 0, bad) != -1
+    bad = wp.copy(); bad[30] = 0                  # leaf claims level 1
+    # its tree neighbour 14 (level 3) now sees a level-1 neighbour and fails first
+    assert oracle.check_bfs(g.offsets, g.adj, 0, bad) == 14
+    bad = wp.copy(); bad[29] = -1                 # reachable vertex left unreached
+    assert oracle.check_bfs(g.offsets, g.adj, 0, bad) != -1
+    bad = wp.copy(); bad[3] = 7; bad[7] = 3       # cycle
+    assert oracle.check_bfs(g.offsets, g.adj, 0, bad) != -1
+    assert oracle.check_bfs(g.offsets, g.adj, 0, wp[:-1]) == -2  # size mismatch
+    g2, wp2, _ = load_golden("pairs12")
+    bad = wp2.copy(); bad[8] = 6                  # unreachable vertex marked reached
+    assert oracle.check_bfs(g2.offsets, g2.adj, 6, bad) != -1
+
+
+def test_max_rule_rejects_min_parent():
+    # The checker accepts any BFS tree; the max rule pins the choice.
+    g, wp, wd = load_golden("layered11")
+    lo = wp.copy()
+    lo[6:10] = 1                                  # smallest level-1 neighbour instead
+    assert oracle.check_bfs(g.offsets, g.adj, 0, lo) == -1
+    assert oracle.max_rule_violations(g.offsets, g.adj, 0, lo, wd) == 4
+    assert oracle.max_rule_violations(g.offsets, g.adj, 0, wp, wd) == 0
+
+
+def test_level_number_rejects_cycle():
+    with pytest.raises(ValueError):
+        oracle.level_number(np.array([0, 2, 1], dtype=np.int32))
