@@ -136,6 +136,19 @@ typedef struct {
 int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
              int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats);
 
+/* ---- batched BFS (up to 64 sources in one traversal) ------------------- */
+#define PBBS_MAX_BATCH 64
+
+/* BFS from each of sources[0..nsrc) (1 <= nsrc <= 64, repeats allowed) in one
+ * traversal that carries a 64-bit source mask per vertex.  parents and depths
+ * are [nsrc][n], source-major: row i is exactly what pbbs_bfs(sources[i])
+ * returns.  Needs a symmetric graph.  Stats: td_levels / bu_levels count push
+ * / pull levels, reached = sum over sources of n_r, edges_reached = sum of m_r.
+ * No reference counterpart: it batches the reference's BFS(start, G)
+ * (bench/BFS.h:11) over many starts. */
+int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
+                   int32_t* parents, int32_t* depths, uint32_t out_flags, pbbs_bfs_stats* stats);
+
 /* Per-level timing of one BFS, for tuning and measurement: every level kernel
  * is launched on its own with a hipEvent after it (no graph replay), so the
  * times exclude launch gaps.  Writes up to `max_levels` records and the number

@@ -100,6 +100,8 @@ SIGNATURES = [
     ("pbbs_graph_info_get", C.c_int, [_P, C.POINTER(GraphInfo)]),
     ("pbbs_graph_free", C.c_int, [_P]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
+    ("pbbs_bfs_batch", C.c_int,
+     [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_profile", C.c_int,
      [_P, _P, C.c_int32, C.POINTER(LevelProfile), C.c_int32, C.POINTER(C.c_int32),
       C.POINTER(C.c_double)]),

@@ -243,6 +243,26 @@ class DeviceGraph:
                              _ptr(dep) if depth else None, 0, C.byref(st)), "BFS")
         return parent, dep, st.as_dict()
 
+    def bfs_batch(self, sources, depth=True):
+        """BFS from each source (at most 64) in one traversal.  Returns
+        (parents[k][n], depths[k][n] or None, stats); row i equals bfs(sources[i])."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        k = src.size
+        parents = np.empty((k, self.n), dtype=np.int32)
+        deps = np.empty((k, self.n), dtype=np.int32) if depth else None
+        st = _lib.BfsStats()
+        check(lib().pbbs_bfs_batch(self.ctx._h, self._h, _ptr(src), k, _ptr(parents),
+                                   _ptr(deps) if depth else None, 0, C.byref(st)), "BFS batch")
+        return parents, deps, st.as_dict()
+
+    def bfs_batch_device(self, sources, parent_ptr=None, depth_ptr=None):
+        """Batched BFS with outputs left in HBM; returns the stats dict."""
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        st = _lib.BfsStats()
+        check(lib().pbbs_bfs_batch(self.ctx._h, self._h, _ptr(src), src.size, parent_ptr,
+                                   depth_ptr, _lib.PBBS_OUT_DEVICE, C.byref(st)), "BFS batch")
+        return st.as_dict()
+
     def profile(self, source, max_levels=4096):
         """Per-level kernel times of one BFS (levels launched one by one).
         Returns (init_ms, [ {level, mode, frontier, frontier_edges, ms}, ... ])."""

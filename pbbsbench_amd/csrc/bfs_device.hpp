@@ -62,6 +62,8 @@ struct BfsArgs {
   BfsState* st;
   uint32_t* wg_level;          // per-workgroup level counter (kMaxGrid entries)
   int32_t* host_status;        // pinned host memory: [0] done, [1] levels started
+  const uint32_t* bu_bounds;   // first-word index of each wavefront's share in the first
+                               // bottom-up level after top-down (num_waves + 1 entries)
   int64_t n;
   int64_t m;
   int64_t nwords;              // ceil(n/64)
@@ -70,6 +72,48 @@ struct BfsArgs {
   int32_t bu_to_td_div;
   int32_t bu_serial_depth;
   int32_t symmetric;
+};
+
+// ----------------------------------------------------------- batched BFS
+// Up to 64 sources in one traversal: bit i of a per-vertex 64-bit mask
+// stands for source i.  F_k of every source is fr[k % 4]; the union frontier
+// (vertices with a non-zero mask) is queued like the single-source one.
+constexpr int kMaxSources = 64;
+
+struct MbfsState {
+  unsigned long long ctr[3][kSubQueues];  // union-frontier sub-queue counters, [k % 3]
+  int32_t modes[4];
+  int32_t finished;
+  int32_t push_levels;
+  int32_t pull_levels;
+  int32_t max_depth;
+  unsigned long long reached_pairs;  // (source, vertex) pairs reached (finalise kernel)
+  unsigned long long edges_pairs;    // sum over those pairs of deg(vertex)
+};
+
+template <typename OffT>
+struct MbfsArgs {
+  const OffT* off;
+  const int32_t* adj;
+  int32_t* parent;             // out: [nsrc][n], source-major
+  int32_t* depth;              // out: [nsrc][n]
+  unsigned long long* vis;     // [n] sources that reached v
+  unsigned long long* fr[4];   // [n] each
+  int32_t* q[2];               // union queue, kSubQueues regions of qcap
+  uint32_t* qe[2];
+  MbfsState* st;
+  uint32_t* wg_level;
+  int32_t* host_status;
+  const int32_t* sources;      // [nsrc]
+  int64_t n;
+  int64_t m;
+  int64_t nwords;
+  int64_t qcap;
+  unsigned long long srcmask;  // bits of the sources in use
+  int32_t nsrc;
+  int32_t td_to_bu_div;
+  int32_t bu_to_td_div;
+  int32_t pad;
 };
 
 }  // namespace pbbs

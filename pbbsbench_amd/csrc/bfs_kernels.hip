@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <hipcub/hipcub.hpp>
+
 #include "bfs_device.hpp"
 
 namespace pbbs {
@@ -113,6 +115,7 @@ struct LevelView {
   unsigned long long* ctr_next;     // its counter
   uint32_t* d0_next;
   const unsigned long long* ctr_cur;  // F_k's counters (all sub-queues)
+  const uint32_t* bu_bounds;        // weighted wavefront shares, or null for equal shares
   long long n;
   long long nwords;
   long long qcap;
@@ -137,8 +140,8 @@ struct Stage {
 
 // Writes buffer entries [0, cnt) of wavefront w to the next frontier at
 // (qbase, ebase) and empties it.
-template <typename OffT>
-__device__ __forceinline__ void stage_copy_out(const LevelView<OffT>& a, StageBuf& sb, int w,
+template <typename View>
+__device__ __forceinline__ void stage_copy_out(const View& a, StageBuf& sb, int w,
                                                unsigned cnt, unsigned qbase, unsigned ebase) {
   for (unsigned i = lane_id(); i < cnt; i += kWave) {
     a.q_next[qbase + i] = sb.v[w][i];
@@ -146,8 +149,8 @@ __device__ __forceinline__ void stage_copy_out(const LevelView<OffT>& a, StageBu
   }
 }
 
-template <typename OffT>
-__device__ __forceinline__ void stage_flush_wave(const LevelView<OffT>& a, StageBuf& sb, Stage& s) {
+template <typename View>
+__device__ __forceinline__ void stage_flush_wave(const View& a, StageBuf& sb, Stage& s) {
   const int w = threadIdx.x / kWave;
   unsigned long long base = 0;
   if (lane_id() == 0)
@@ -161,9 +164,9 @@ __device__ __forceinline__ void stage_flush_wave(const LevelView<OffT>& a, Stage
 }
 
 // Stages the lanes with `take` (vertex v of degree deg) for F_{k+1}.
-template <typename OffT>
-__device__ __forceinline__ void stage_push(const LevelView<OffT>& a, StageBuf& sb, Stage& s,
-                                           bool take, int v, uint32_t deg) {
+template <typename View>
+__device__ __forceinline__ void stage_push(const View& a, StageBuf& sb, Stage& s, bool take, int v,
+                                           uint32_t deg) {
   const unsigned long long mask = __ballot(take);
   if (mask == 0) return;
   const unsigned c = (unsigned)__popcll(mask);
@@ -181,9 +184,8 @@ __device__ __forceinline__ void stage_push(const LevelView<OffT>& a, StageBuf& s
 }
 
 // End of level: one reservation for the whole workgroup, then copy-out.
-template <typename OffT>
-__device__ __forceinline__ void stage_flush_block(const LevelView<OffT>& a, StageBuf& sb,
-                                                  const Stage& s) {
+template <typename View>
+__device__ __forceinline__ void stage_flush_block(const View& a, StageBuf& sb, const Stage& s) {
   const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
     sb.cnt[w] = s.cnt;
@@ -520,7 +522,8 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
       reinterpret_cast<const unsigned long long*>(a.fb_cur);
   const long long W = num_waves();
   const long long gw = global_wave();
-  const long long b0 = a.nwords * gw / W, b1 = a.nwords * (gw + 1) / W;
+  const long long b0 = a.bu_bounds ? (long long)a.bu_bounds[gw] : a.nwords * gw / W;
+  const long long b1 = a.bu_bounds ? (long long)a.bu_bounds[gw + 1] : a.nwords * (gw + 1) / W;
   unsigned nq = 0, npark = 0;
   for (long long g0 = b0; g0 < b1; g0 += kWave) {
     const long long mb = g0 + lane;
@@ -661,6 +664,11 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.ctr_next = &st->ctr[c1][sq];
   a.d0_next = &st->d0[c1];
   a.ctr_cur = st->ctr[c0];
+  // The first bottom-up level after top-down meets almost every vertex
+  // unreached; its per-vertex cost grows with the degree, so the shares are
+  // cut at equal sums of 1 + log2(1 + degree).  Later bottom-up levels meet
+  // few, low-degree vertices and split the words evenly.
+  a.bu_bounds = (prev == kTopDown) ? A->bu_bounds : nullptr;
   a.n = A->n;
   a.nwords = A->nwords;
   a.k = k;
@@ -721,6 +729,399 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
     __threadfence_system();
     __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// ---------------------------------------------------------------- batched BFS
+// Up to 64 sources share one traversal.  Bit i of a per-vertex 64-bit mask
+// stands for source i; F_k of all sources is the mask array fr[k % 4] and the
+// vertices with a non-zero mask form the union frontier queue.  Each source's
+// outputs follow the single-source rule (largest neighbour one level up), so
+// a batch returns exactly what separate BFS calls return.
+//
+//   * push level k: every edge (u,v) of the union frontier ORs
+//     fr_k[u] & ~reached(v) into fr_{k+1}[v]; the lane whose atomicOr saw 0
+//     appends v.  The parents of those bits are found in level k+1's kernel
+//     by scanning v's sorted row from the top against fr_k (symmetric graph:
+//     the row holds v's in-neighbours), clearing each bit at its first hit.
+//   * pull level k: every vertex with sources still unreached scans its row
+//     from the top against fr_k, assigning parents as bits hit, until all of
+//     its unreached bits are found or the row ends.
+// "Reached" bits of v during level k are vis[v] | fr_k[v]: vis gets level-k
+// bits during level k (resolve), so the union keeps the guard exact.
+template <typename OffT>
+struct MbfsView {
+  const OffT* __restrict__ off;
+  const int32_t* __restrict__ adj;
+  int32_t* parent;
+  int32_t* depth;
+  unsigned long long* vis;
+  const unsigned long long* fr_prev;   // F_{k-1}
+  const unsigned long long* fr_cur;    // F_k
+  unsigned long long* fr_next;         // F_{k+1}
+  const int32_t* q_cur;
+  const uint32_t* qe_cur;
+  const unsigned long long* ctr_cur;
+  int32_t* q_next;                     // this workgroup's sub-queue region
+  uint32_t* qe_next;
+  unsigned long long* ctr_next;
+  long long n;
+  long long nwords;
+  long long qcap;
+  unsigned long long srcmask;
+  int k;
+};
+
+template <typename OffT>
+__device__ __forceinline__ void assign_sources(const MbfsView<OffT>& a, int v, int u,
+                                               unsigned long long bits, int d) {
+  while (bits) {
+    const long long i = __builtin_ctzll(bits);
+    bits &= bits - 1;
+    a.parent[i * a.n + v] = u;
+    a.depth[i * a.n + v] = d;
+  }
+}
+
+// Parents of the bits F_k[v] for the queue entries of F_k (found by a push).
+template <typename OffT>
+__device__ void mbfs_resolve(const MbfsView<OffT>& a, unsigned long long total) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)total;
+       t += stride) {
+    // entry t of the concatenated sub-queues
+    long long idx = t;
+    int sq = 0;
+    for (; sq < kSubQueues; ++sq) {
+      const long long c = (uint32_t)a.ctr_cur[sq];
+      if (idx < c) break;
+      idx -= c;
+    }
+    const int v = a.q_cur[sq * a.qcap + idx];
+    unsigned long long rem = a.fr_cur[v];
+    const unsigned long long bits = rem;
+    OffT lo = a.off[v], p = a.off[v + 1];
+    while (rem && p > lo) {
+      const OffT have = p - lo;
+      int c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
+      unsigned long long mk[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) mk[j] = c[j] >= 0 ? a.fr_prev[c[j]] : 0ull;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const unsigned long long hit = mk[j] & rem;
+        if (hit) {
+          assign_sources(a, v, c[j], hit, a.k);
+          rem &= ~hit;
+        }
+      }
+      p = have > 4 ? p - 4 : lo;
+    }
+    atomicOr(&a.vis[v], bits);
+  }
+}
+
+template <typename OffT>
+__device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
+                          unsigned long long E) {
+  const unsigned lane = lane_id();
+  const unsigned long long W = (unsigned long long)num_waves();
+  const unsigned long long gw = (unsigned long long)global_wave();
+  const unsigned long long ebeg = E * gw / W, eend = E * (gw + 1) / W;
+  if (ebeg >= eend) return;
+  unsigned long long P = 0;
+  for (int sq = 0; sq < kSubQueues; ++sq) {
+    const unsigned long long cc = a.ctr_cur[sq];
+    const unsigned long long Es = cc >> 32;
+    const unsigned nf = (unsigned)cc;
+    const unsigned long long lo_e = ebeg > P ? ebeg : P;
+    const unsigned long long hi_e = eend < P + Es ? eend : P + Es;
+    P += Es;
+    if (lo_e >= hi_e) continue;
+    const int32_t* __restrict__ q = a.q_cur + sq * a.qcap;
+    const uint32_t* __restrict__ qe = a.qe_cur + sq * a.qcap;
+    const uint32_t eb = (uint32_t)(lo_e - (P - Es)), ee = (uint32_t)(hi_e - (P - Es));
+    unsigned lo = 0, hi = nf;
+    while (hi - lo > (unsigned)kWave) {
+      const unsigned step = (hi - lo + kWave - 1) / kWave;
+      const unsigned idx = lo + lane * step;
+      const bool pr = idx < hi && qe[idx] <= eb;
+      const unsigned cnt = (unsigned)__popcll(__ballot(pr));
+      lo += (cnt - 1) * step;
+      hi = min(hi, lo + step);
+    }
+    unsigned i;
+    {
+      const unsigned idx = lo + lane;
+      const bool pr = idx < hi && qe[idx] <= eb;
+      i = lo + (unsigned)__popcll(__ballot(pr)) - 1;
+    }
+    uint32_t e = eb;
+    while (e < ee) {
+      const unsigned idx = i + lane;
+      int32_t qv = 0;
+      uint32_t qo = 0xffffffffu;
+      OffT rs = 0;
+      unsigned long long qm = 0;
+      if (idx < nf) {
+        qv = q[idx];
+        qo = qe[idx];
+        rs = a.off[qv];
+        qm = a.fr_cur[qv];
+      }
+      const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : (uint32_t)Es;
+      const uint32_t lim = min(wend, ee);
+      for (uint32_t base = e; base < lim; base += kWave) {
+        const uint32_t my = base + lane;
+        const bool valid = (unsigned long long)base + lane < (unsigned long long)lim;
+        int j = 0;
+#pragma unroll
+        for (int st = kWave / 2; st >= 1; st >>= 1) {
+          const uint32_t qt = __shfl(qo, j + st);
+          if (qt <= my) j += st;
+        }
+        const OffT rsj = shfl_off(rs, j);
+        const uint32_t qoj = __shfl(qo, j);
+        const unsigned long long um = __shfl(qm, j);
+        bool fresh = false;
+        int v = 0;
+        if (valid) {
+          v = a.adj[rsj + (OffT)(my - qoj)];
+          const unsigned long long nb = um & ~(a.vis[v] | a.fr_cur[v]) & a.srcmask;
+          if (nb) fresh = atomicOr(&a.fr_next[v], nb) == 0ull;
+        }
+        if (__ballot(fresh)) {
+          uint32_t deg = 0;
+          if (fresh) deg = (uint32_t)(a.off[v + 1] - a.off[v]);
+          stage_push(a, sb, stg, fresh, v, deg);
+        }
+      }
+      e = lim;
+      i += kWave;
+    }
+  }
+}
+
+template <typename OffT>
+__device__ void mbfs_pull(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg) {
+  const unsigned lane = lane_id();
+  const long long W = num_waves();
+  const long long gw = global_wave();
+  const long long b0 = a.nwords * gw / W, b1 = a.nwords * (gw + 1) / W;
+  for (long long b = b0; b < b1; ++b) {
+    const long long v = b * kWave + lane;
+    unsigned long long rem = 0;
+    if (v < a.n) rem = ~(a.vis[v] | a.fr_cur[v]) & a.srcmask;
+    if (!__ballot(rem != 0)) continue;
+    unsigned long long found = 0;
+    OffT lo = 0, hi = 0;
+    if (rem) {
+      lo = a.off[v];
+      hi = a.off[v + 1];
+      OffT p = hi;
+      while (rem && p > lo) {
+        const OffT have = p - lo;
+        int c[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
+        unsigned long long mk[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) mk[j] = c[j] >= 0 ? a.fr_cur[c[j]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const unsigned long long hit = mk[j] & rem;
+          if (hit) {
+            assign_sources(a, (int)v, c[j], hit, a.k + 1);
+            found |= hit;
+            rem &= ~hit;
+          }
+        }
+        p = have > 4 ? p - 4 : lo;
+      }
+      if (found) {
+        a.fr_next[v] = found;
+        atomicOr(&a.vis[v], found);
+      }
+    }
+    stage_push(a, sb, stg, found != 0, (int)v, (uint32_t)(hi - lo));
+  }
+}
+
+template <typename OffT>
+__device__ __forceinline__ int mbfs_mode(const MbfsArgs<OffT>& A, int prev, unsigned long long cnt,
+                                         unsigned long long E) {
+  if (A.td_to_bu_div <= 0) return kTopDown;
+  if (prev == kTopDown)
+    return (cnt + E > (unsigned long long)A.m / (unsigned)A.td_to_bu_div) ? kBottomUp : kTopDown;
+  return ((long long)cnt > A.n / A.bu_to_td_div) ? kBottomUp : kTopDown;
+}
+
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>* __restrict__ A) {
+  __shared__ StageBuf sb;
+  MbfsState* st = A->st;
+  const int k = (int)A->wg_level[blockIdx.x];
+  const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
+  unsigned long long cnt = 0, E = 0;
+#pragma unroll
+  for (int s = 0; s < kSubQueues; ++s) {
+    const unsigned long long c = st->ctr[c0][s];
+    cnt += (uint32_t)c;
+    E += c >> 32;
+  }
+  if (cnt == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->finished) {
+      st->finished = 1;
+      __threadfence_system();
+      __hip_atomic_store(&A->host_status[1], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(&A->host_status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+    return;
+  }
+  const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
+  const int mode = mbfs_mode(*A, prev, cnt, E);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->modes[k & 3] = mode;
+    st->max_depth = k;
+    if (mode == kTopDown) st->push_levels += 1; else st->pull_levels += 1;
+#pragma unroll
+    for (int s = 0; s < kSubQueues; ++s) st->ctr[c2][s] = 0;
+    __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+  {
+    // F_{k+2}'s masks held F_{k-2}: level k reads F_{k-1} and F_k only.
+    unsigned long long* z = A->fr[(k + 2) & 3];
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < A->n; v += stride)
+      z[v] = 0;
+  }
+  const int sq = blockIdx.x % kSubQueues;
+  MbfsView<OffT> a;
+  a.off = A->off;
+  a.adj = A->adj;
+  a.parent = A->parent;
+  a.depth = A->depth;
+  a.vis = A->vis;
+  a.fr_prev = A->fr[(k + 3) & 3];
+  a.fr_cur = A->fr[k & 3];
+  a.fr_next = A->fr[(k + 1) & 3];
+  a.qcap = A->qcap;
+  a.q_cur = A->q[k & 1];
+  a.qe_cur = A->qe[k & 1];
+  a.ctr_cur = st->ctr[c0];
+  a.q_next = A->q[(k + 1) & 1] + sq * a.qcap;
+  a.qe_next = A->qe[(k + 1) & 1] + sq * a.qcap;
+  a.ctr_next = &st->ctr[c1][sq];
+  a.n = A->n;
+  a.nwords = A->nwords;
+  a.srcmask = A->srcmask;
+  a.k = k;
+  if (k > 0 && prev == kTopDown) mbfs_resolve(a, cnt);
+  Stage stg;
+  if (mode == kTopDown)
+    mbfs_push(a, sb, stg, E);
+  else
+    mbfs_pull(a, sb, stg);
+  stage_flush_block(a, sb, stg);
+  if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
+}
+
+// Clears the masks and presets every output to "unreached".
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nth = (long long)gridDim.x * blockDim.x;
+  for (long long v = tid; v < a.n; v += nth) {
+    a.vis[v] = 0;
+    a.fr[0][v] = 0;
+    a.fr[1][v] = 0;
+    a.fr[2][v] = 0;
+    a.fr[3][v] = 0;
+  }
+  const long long total = (long long)a.nsrc * a.n;
+  for (long long t = tid; t < total; t += nth) {
+    a.parent[t] = -1;
+    a.depth[t] = (int32_t)a.n;
+  }
+  for (long long i = tid; i < kMaxGrid; i += nth) a.wg_level[i] = 0;
+}
+
+// One workgroup: marks the sources, queues the distinct ones as F_0 and
+// resets the control block.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, MbfsArgs<OffT>* dst) {
+  __shared__ int first[kMaxSources];
+  const int t = threadIdx.x;
+  if (t < a.nsrc) {
+    const int s = a.sources[t];
+    const unsigned long long bit = 1ull << t;
+    a.parent[(long long)t * a.n + s] = s;
+    a.depth[(long long)t * a.n + s] = 0;
+    atomicOr(&a.vis[s], bit);
+    first[t] = atomicOr(&a.fr[0][s], bit) == 0ull;
+  }
+  __syncthreads();
+  if (t == 0) {
+    *dst = a;
+    MbfsState* st = a.st;
+    unsigned cnt = 0;
+    unsigned long long E = 0;
+    for (int i = 0; i < a.nsrc; ++i) {
+      if (!first[i]) continue;
+      const int s = a.sources[i];
+      a.q[0][cnt] = s;
+      a.qe[0][cnt] = (uint32_t)E;
+      E += (unsigned long long)(a.off[s + 1] - a.off[s]);
+      ++cnt;
+    }
+    for (int j = 0; j < 3; ++j)
+      for (int q = 0; q < kSubQueues; ++q) st->ctr[j][q] = 0;
+    st->ctr[0][0] = (E << 32) | cnt;
+    for (int j = 0; j < 4; ++j) st->modes[j] = kTopDown;
+    st->finished = 0;
+    st->push_levels = 0;
+    st->pull_levels = 0;
+    st->max_depth = 0;
+    st->reached_pairs = 0;
+    st->edges_pairs = 0;
+    __threadfence_system();
+    __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Totals over (source, vertex) pairs reached: count and degree sum.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mbfs_stats_kernel(MbfsArgs<OffT> a) {
+  __shared__ unsigned long long sp[kWavesPerBlock], se[kWavesPerBlock];
+  unsigned long long pairs = 0, edges = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += stride) {
+    const unsigned long long c = (unsigned long long)__popcll(a.vis[v] & a.srcmask);
+    pairs += c;
+    edges += c * (unsigned long long)(a.off[v + 1] - a.off[v]);
+  }
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    pairs += __shfl_xor(pairs, d);
+    edges += __shfl_xor(edges, d);
+  }
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+    sp[w] = pairs;
+    se[w] = edges;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long P = 0, Ed = 0;
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      P += sp[i];
+      Ed += se[i];
+    }
+    atomicAdd(&a.st->reached_pairs, P);
+    atomicAdd(&a.st->edges_pairs, Ed);
   }
 }
 
@@ -835,6 +1236,43 @@ __global__ __launch_bounds__(kBlock) void graph_symmetry_kernel(
   }
 }
 
+// ------------------------------------------------------ bottom-up shares
+// Per frontier word: sum over its vertices of degree 0 ? 0 : 1 + floor(log2(1 + deg)).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void word_weight_kernel(const OffT* __restrict__ off,
+                                                             long long n, long long nwords,
+                                                             unsigned long long* __restrict__ ww) {
+  const unsigned lane = lane_id();
+  for (long long b = global_wave(); b < nwords; b += num_waves()) {
+    const long long v = b * kWave + lane;
+    unsigned x = 0;
+    if (v < n) {
+      const unsigned long long deg = (unsigned long long)(off[v + 1] - off[v]);
+      if (deg) x = 1u + (63u - (unsigned)__builtin_clzll(deg + 1));
+    }
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) x += __shfl_xor(x, d);
+    if (lane == 0) ww[b] = x;
+  }
+}
+
+// bounds[i] = first word j with prefix[j] >= total * i / waves (prefix has
+// nwords + 1 entries, exclusive).
+__global__ __launch_bounds__(kBlock) void share_bounds_kernel(const unsigned long long* __restrict__ prefix,
+                                                              long long nwords, long long waves,
+                                                              uint32_t* __restrict__ bounds) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > waves) return;
+  const unsigned long long total = prefix[nwords];
+  const unsigned long long target = (unsigned long long)((__uint128_t)total * i / waves);
+  long long lo = 0, hi = nwords;  // smallest j in [0, nwords] with prefix[j] >= target
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (prefix[mid] >= target) hi = mid; else lo = mid + 1;
+  }
+  bounds[i] = (uint32_t)(i == waves ? nwords : lo);
+}
+
 // ------------------------------------------------------------ host launchers
 template <typename OffT>
 hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
@@ -890,6 +1328,65 @@ hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long 
   return hipGetLastError();
 }
 
+template <typename OffT>
+hipError_t launch_word_weights(const OffT* off, long long n, unsigned long long* ww, int grid,
+                               hipStream_t stream) {
+  const long long nwords = (n + kWave - 1) / kWave;
+  hipLaunchKernelGGL(word_weight_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, n, nwords,
+                     ww);
+  return hipGetLastError();
+}
+
+// out[i] = sum of in[0..i) for i <= n (in has n + 1 entries, in[n] ignored).
+hipError_t exclusive_scan_u64(const unsigned long long* in, unsigned long long* out, long long n,
+                              hipStream_t stream) {
+  size_t bytes = 0;
+  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)(n + 1), stream);
+  if (e != hipSuccess) return e;
+  void* tmp = nullptr;
+  e = hipMalloc(&tmp, bytes);
+  if (e != hipSuccess) return e;
+  e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)(n + 1), stream);
+  const hipError_t s = hipStreamSynchronize(stream);
+  (void)hipFree(tmp);
+  return e != hipSuccess ? e : s;
+}
+
+hipError_t launch_share_bounds(const unsigned long long* prefix, long long nwords, long long waves,
+                               uint32_t* bounds, hipStream_t stream) {
+  const long long threads = waves + 1;
+  hipLaunchKernelGGL(share_bounds_kernel, dim3((unsigned)((threads + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, stream, prefix, nwords, waves, bounds);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_mbfs_setup(const MbfsArgs<OffT>& a, MbfsArgs<OffT>* dst, int grid,
+                             hipStream_t stream) {
+  hipLaunchKernelGGL(mbfs_clear_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(mbfs_seed_kernel<OffT>, dim3(1), dim3(kBlock), 0, stream, a, dst);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+const void* mbfs_level_symbol() {
+  return reinterpret_cast<const void*>(&mbfs_level_kernel<OffT>);
+}
+
+template <typename OffT>
+hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(mbfs_level_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, dargs);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(mbfs_stats_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, a);
+  return hipGetLastError();
+}
+
 #ifdef PBBS_WAVE_TIMING
 extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
   const long long cap = (long long)kTimingLevels * kTimingWaves * 3;
@@ -912,6 +1409,13 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
                                                  hipStream_t);                                   \
   template hipError_t launch_graph_rows<OffT>(const OffT*, const int32_t*, long long, long long, \
                                               unsigned long long*, int, hipStream_t);            \
+  template hipError_t launch_mbfs_setup<OffT>(const MbfsArgs<OffT>&, MbfsArgs<OffT>*, int,      \
+                                             hipStream_t);                                      \
+  template const void* mbfs_level_symbol<OffT>();                                               \
+  template hipError_t launch_mbfs_level<OffT>(const MbfsArgs<OffT>*, int, hipStream_t);          \
+  template hipError_t launch_mbfs_stats<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
+  template hipError_t launch_word_weights<OffT>(const OffT*, long long, unsigned long long*, int, \
+                                               hipStream_t);                                    \
   template hipError_t launch_graph_symmetry<OffT>(const OffT*, const int32_t*, long long,        \
                                                   long long, unsigned long long*, int,           \
                                                   hipStream_t);

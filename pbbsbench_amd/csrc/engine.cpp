@@ -34,6 +34,22 @@ template <typename OffT>
 hipError_t launch_graph_rows(const OffT* off, const int32_t* adj, long long n, long long m,
                              unsigned long long* counters, int grid, hipStream_t stream);
 template <typename OffT>
+hipError_t launch_mbfs_setup(const MbfsArgs<OffT>& a, MbfsArgs<OffT>* dst, int grid,
+                             hipStream_t stream);
+template <typename OffT>
+const void* mbfs_level_symbol();
+template <typename OffT>
+hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_word_weights(const OffT* off, long long n, unsigned long long* ww, int grid,
+                               hipStream_t stream);
+hipError_t exclusive_scan_u64(const unsigned long long* in, unsigned long long* out, long long n,
+                              hipStream_t stream);
+hipError_t launch_share_bounds(const unsigned long long* prefix, long long nwords, long long waves,
+                               uint32_t* bounds, hipStream_t stream);
+template <typename OffT>
 hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long n, long long m,
                                  unsigned long long* counters, int grid, hipStream_t stream);
 
@@ -85,6 +101,16 @@ struct pbbs_ctx {
   BfsState* host_state = nullptr;        // pinned: stats read-back
   hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> level_graphs;
+  // batched BFS
+  int64_t cap_bn = 0;                    // vertices the mask buffers hold
+  unsigned long long* b_masks = nullptr; // vis | fr0 | fr1 | fr2 | fr3
+  MbfsState* b_state = nullptr;
+  void* b_args = nullptr;                // device copy of MbfsArgs<OffT>
+  int32_t* b_sources = nullptr;          // kMaxSources
+  int64_t cap_bout = 0;                  // int32 entries of each output buffer
+  int32_t* b_parent = nullptr;
+  int32_t* b_depth = nullptr;
+  std::map<std::tuple<int, int, int>, hipGraphExec_t> batch_graphs;
 };
 
 struct pbbs_graph {
@@ -96,6 +122,9 @@ struct pbbs_graph {
   void* d_off = nullptr;
   int32_t* d_adj = nullptr;
   unsigned long long* d_iso = nullptr;  // degree-0 bitmap
+  unsigned long long* d_wprefix = nullptr;  // exclusive prefix of per-word degree weights
+  uint32_t* d_bounds = nullptr;         // wavefront shares of the weighted bottom-up split
+  int64_t bounds_waves = 0;             // wavefront count d_bounds was cut for
   int64_t isolated = 0;
   int64_t max_degree = 0;
   int64_t device_bytes = 0;
@@ -182,7 +211,14 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& kv : c->level_graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second);
   free_workspace(c);
+  (void)hipFree(c->b_masks);
+  (void)hipFree(c->b_state);
+  (void)hipFree(c->b_args);
+  (void)hipFree(c->b_sources);
+  (void)hipFree(c->b_parent);
+  (void)hipFree(c->b_depth);
   (void)hipFree(c->w_state);
   (void)hipFree(c->w_wg_level);
   (void)hipFree(c->w_args);
@@ -274,8 +310,23 @@ static int upload_checks(pbbs_ctx* c, pbbs_graph* g, uint32_t flags) {
     } else {
       g->symmetric = false;
     }
+    // per-word degree weights and their prefix, for the weighted bottom-up split
+    unsigned long long* ww = nullptr;
+    if (hipMalloc(&ww, (nwords + 1) * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&g->d_wprefix, (nwords + 1) * sizeof(unsigned long long)) != hipSuccess) {
+      (void)hipFree(ww);
+      rc = fail(PBBS_ERR_NOMEM, "weight prefix allocation failed");
+      break;
+    }
+    const hipError_t we = launch_word_weights<OffT>(off, g->n, ww, grid, s);
+    const hipError_t se = we == hipSuccess ? exclusive_scan_u64(ww, g->d_wprefix, nwords, s) : we;
+    (void)hipFree(ww);
+    if (se != hipSuccess) {
+      rc = fail(PBBS_ERR_HIP, "weight prefix failed: %s", hipGetErrorString(se));
+      break;
+    }
+    g->device_bytes += (nwords + 1) * sizeof(unsigned long long);
   } while (false);
-  (void)nwords;
   (void)hipFree(d_cnt);
   return rc;
 }
@@ -291,6 +342,8 @@ int pbbs_graph_free(pbbs_graph* g) {
   (void)hipFree(g->d_off);
   (void)hipFree(g->d_adj);
   (void)hipFree(g->d_iso);
+  (void)hipFree(g->d_wprefix);
+  (void)hipFree(g->d_bounds);
   delete g;
   return PBBS_OK;
 }
@@ -400,6 +453,20 @@ static int next_chunk(int base, int launched) {
   return k;
 }
 
+// Cuts the weighted bottom-up shares for `waves` wavefronts (cached per graph).
+static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves) {
+  if (g->bounds_waves == waves) return PBBS_OK;
+  (void)hipFree(g->d_bounds);
+  g->d_bounds = nullptr;
+  g->bounds_waves = 0;
+  const int64_t nwords = (g->n + kWave - 1) / kWave;
+  HIP_TRY(hipMalloc(&g->d_bounds, (waves + 1) * sizeof(uint32_t)));
+  HIP_TRY(launch_share_bounds(g->d_wprefix, nwords, waves, g->d_bounds, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  g->bounds_waves = waves;
+  return PBBS_OK;
+}
+
 template <typename OffT>
 static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth) {
   const int64_t nwords = (g->n + kWave - 1) / kWave;
@@ -421,6 +488,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.st = c->w_state;
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
+  a.bu_bounds = g->d_bounds;
   a.n = g->n;
   a.m = g->m;
   a.nwords = nwords;
@@ -434,8 +502,10 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
 template <typename OffT>
 static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
                    pbbs_bfs_stats* stats) {
-  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth);
   const int grid = level_grid(c, g->off64, g->sorted);
+  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
+  if (brc != PBBS_OK) return brc;
+  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   volatile int32_t* hs = c->host_status;
@@ -528,8 +598,10 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
 template <typename OffT>
 static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
                        int32_t max_levels, int32_t* count, double* init_ms) {
-  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth);
   const int grid = level_grid(c, g->off64, g->sorted);
+  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
+  if (brc != PBBS_OK) return brc;
+  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   volatile int32_t* hs = c->host_status;
@@ -613,4 +685,205 @@ extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
   if (rc != PBBS_OK) return rc;
   return g->off64 ? run_profile<int64_t>(c, g, source, out, max_levels, count, init_ms)
                   : run_profile<uint32_t>(c, g, source, out, max_levels, count, init_ms);
+}
+
+// --------------------------------------------------------------- batched BFS
+static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, bool need_out) {
+  if (!c->b_state) {
+    HIP_TRY(hipMalloc(&c->b_state, sizeof(MbfsState)));
+    HIP_TRY(hipMalloc(&c->b_args, sizeof(MbfsArgs<int64_t>)));
+    HIP_TRY(hipMalloc(&c->b_sources, kMaxSources * sizeof(int32_t)));
+  }
+  if (n > c->cap_bn) {
+    (void)hipFree(c->b_masks);
+    c->b_masks = nullptr;
+    c->cap_bn = 0;
+    if (hipMalloc(&c->b_masks, 5 * n * sizeof(unsigned long long)) != hipSuccess)
+      return fail(PBBS_ERR_NOMEM, "batched mask buffers for n=%lld failed", (long long)n);
+    c->cap_bn = n;
+  }
+  if (need_out && out_entries > c->cap_bout) {
+    (void)hipFree(c->b_parent);
+    (void)hipFree(c->b_depth);
+    c->b_parent = c->b_depth = nullptr;
+    c->cap_bout = 0;
+    if (hipMalloc(&c->b_parent, out_entries * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->b_depth, out_entries * sizeof(int32_t)) != hipSuccess)
+      return fail(PBBS_ERR_NOMEM, "batched output buffers (%lld entries) failed",
+                  (long long)out_entries);
+    c->cap_bout = out_entries;
+  }
+  return PBBS_OK;
+}
+
+static int batch_grid(pbbs_ctx* c, bool off64) {
+  if (c->tuning.grid_blocks > 0) return std::min(c->tuning.grid_blocks, kMaxGrid);
+  int per_cu = 0;
+  const void* fn = off64 ? mbfs_level_symbol<int64_t>() : mbfs_level_symbol<uint32_t>();
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return std::min(per_cu * c->num_cus, kMaxGrid);
+}
+
+template <typename OffT>
+static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out) {
+  const auto key = std::make_tuple((int)sizeof(OffT), grid, K);
+  auto it = c->batch_graphs.find(key);
+  if (it != c->batch_graphs.end()) {
+    *out = it->second;
+    return PBBS_OK;
+  }
+  hipGraph_t graph = nullptr;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t le = hipSuccess;
+  for (int i = 0; i < K && le == hipSuccess; ++i)
+    le = launch_mbfs_level<OffT>(static_cast<const MbfsArgs<OffT>*>(c->b_args), grid, c->stream);
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(PBBS_ERR_HIP, "capturing %d batched level launches failed: %s", K,
+                hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) return fail(PBBS_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+  c->batch_graphs[key] = exec;
+  *out = exec;
+  return PBBS_OK;
+}
+
+template <typename OffT>
+static int run_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent, int32_t* d_depth,
+                     pbbs_bfs_stats* stats) {
+  const int64_t n = g->n;
+  MbfsArgs<OffT> a;
+  std::memset(&a, 0, sizeof a);
+  a.off = static_cast<const OffT*>(g->d_off);
+  a.adj = g->d_adj;
+  a.parent = d_parent;
+  a.depth = d_depth;
+  a.vis = c->b_masks;
+  for (int j = 0; j < 4; ++j) a.fr[j] = c->b_masks + (j + 1) * n;
+  a.q[0] = c->w_q;
+  a.q[1] = c->w_q + kSubQueues * c->cap_n;
+  a.qe[0] = c->w_qe;
+  a.qe[1] = c->w_qe + kSubQueues * c->cap_n;
+  a.qcap = c->cap_n;
+  a.st = c->b_state;
+  a.wg_level = c->w_wg_level;
+  a.host_status = c->host_status;
+  a.sources = c->b_sources;
+  a.n = n;
+  a.m = g->m;
+  a.nwords = (n + kWave - 1) / kWave;
+  a.nsrc = nsrc;
+  a.srcmask = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
+  a.td_to_bu_div = c->tuning.td_to_bu_div;
+  a.bu_to_td_div = c->tuning.bu_to_td_div;
+
+  const int grid = batch_grid(c, g->off64);
+  const int setup_grid =
+      (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, c->num_cus * 16));
+  volatile int32_t* hs = c->host_status;
+  hs[0] = 0;
+  hs[1] = 0;
+  int K = c->tuning.chunk_levels;
+  hipGraphExec_t exec = nullptr;
+  if (c->tuning.use_graphs) {
+    const int rc = get_batch_graph<OffT>(c, grid, K, &exec);
+    if (rc != PBBS_OK) return rc;
+  }
+  HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  HIP_TRY(launch_mbfs_setup<OffT>(a, static_cast<MbfsArgs<OffT>*>(c->b_args), setup_grid,
+                                  c->stream));
+  int64_t launched = 0;
+  for (;;) {
+    if (c->tuning.use_graphs) {
+      if (!exec) {
+        const int rc = get_batch_graph<OffT>(c, grid, K, &exec);
+        if (rc != PBBS_OK) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(exec, c->stream));
+      exec = nullptr;
+    } else {
+      for (int i = 0; i < K; ++i)
+        HIP_TRY(launch_mbfs_level<OffT>(static_cast<const MbfsArgs<OffT>*>(c->b_args), grid,
+                                        c->stream));
+    }
+    launched += K;
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (hs[0]) break;
+    if (launched > n + 2)
+      return fail(PBBS_ERR_STATE, "batched BFS did not finish after %lld level launches",
+                  (long long)launched);
+    K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(launched, 1 << 20));
+  }
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
+  if (stats) {
+    HIP_TRY(launch_mbfs_stats<OffT>(a, c->num_cus * 4, c->stream));
+    MbfsState hst;
+    HIP_TRY(hipMemcpyAsync(&hst, c->b_state, sizeof hst, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memset(stats, 0, sizeof *stats);
+    stats->levels = hst.max_depth + 1;
+    stats->td_levels = hst.push_levels;
+    stats->bu_levels = hst.pull_levels;
+    stats->launches = (int32_t)launched;
+    stats->reached = (int64_t)hst.reached_pairs;
+    stats->edges_reached = (int64_t)hst.edges_pairs;
+    stats->device_ms = ms;
+  }
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
+                              int32_t* parents, int32_t* depths, uint32_t out_flags,
+                              pbbs_bfs_stats* stats) {
+  clear_error();
+  if (!c || !g || !sources) return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (nsrc < 1 || nsrc > kMaxSources)
+    return fail(PBBS_ERR_INVALID, "source count %d outside [1, %d]", nsrc, kMaxSources);
+  for (int i = 0; i < nsrc; ++i)
+    if (sources[i] < 0 || (int64_t)sources[i] >= g->n)
+      return fail(PBBS_ERR_INVALID, "source %d (%d) outside [0, %lld)", i, sources[i],
+                  (long long)g->n);
+  if (!g->symmetric)
+    return fail(PBBS_ERR_INVALID,
+                "batched BFS needs a symmetric graph (parents are found by scanning each "
+                "vertex's row as its in-neighbours)");
+  HIP_TRY(hipSetDevice(c->device));
+  const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
+  const int64_t entries = (int64_t)nsrc * g->n;
+  int rc = ensure_workspace(c, g->n);
+  if (rc == PBBS_OK) rc = ensure_batch_workspace(c, g->n, entries, !(dev_out && parents && depths));
+  if (rc != PBBS_OK) return rc;
+  int32_t* dp = (dev_out && parents) ? parents : c->b_parent;
+  int32_t* dd = (dev_out && depths) ? depths : c->b_depth;
+  HIP_TRY(hipMemcpyAsync(c->b_sources, sources, nsrc * sizeof(int32_t), hipMemcpyHostToDevice,
+                         c->stream));
+  rc = g->off64 ? run_batch<int64_t>(c, g, nsrc, dp, dd, stats)
+                : run_batch<uint32_t>(c, g, nsrc, dp, dd, stats);
+  if (rc != PBBS_OK) return rc;
+  if (!dev_out && (parents || depths)) {
+    HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
+    if (parents)
+      HIP_TRY(hipMemcpyAsync(parents, dp, entries * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    if (depths)
+      HIP_TRY(hipMemcpyAsync(depths, dd, entries * sizeof(int32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_c1));
+    if (stats) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1));
+      stats->copy_ms = ms;
+    }
+  }
+  return PBBS_OK;
 }
