@@ -149,6 +149,51 @@ int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
 int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
                    int32_t* parents, int32_t* depths, uint32_t out_flags, pbbs_bfs_stats* stats);
 
+/* ---- serving loop ------------------------------------------------------ */
+/* Graphs stay resident; queries arrive from any host thread and complete
+ * asynchronously.  A worker thread keeps up to `slots` runs in flight (each
+ * with its own stream and workspace, so their kernels overlap on the GPU),
+ * admits queries as slots and the HBM budget free up, and merges waiting
+ * queries for one graph into one batched traversal of up to `max_batch`
+ * sources.  Every result is bit-identical to a lone pbbs_bfs call.  No
+ * reference counterpart (the reference runs one BFS per process,
+ * bench/BFSTime.C:51-61). */
+typedef struct pbbs_server pbbs_server;
+
+typedef struct {
+    int32_t slots;        /* concurrent runs; 0 = 4 */
+    int32_t max_batch;    /* queries merged per run, 1..64; 0 = 64; 1 = never merge */
+    int64_t hbm_budget;   /* workspace bytes of the runs in flight at once (a run is
+                             admitted, or its batch shrunk, to fit); 0 = 80% of free HBM */
+    int32_t reserved[4];
+} pbbs_server_config;
+
+typedef struct {
+    uint64_t ticket;
+    int32_t  graph;
+    int32_t  source;
+    int32_t  status;      /* PBBS_OK or the error of the run that served it */
+    int32_t  batch;       /* queries served by that run */
+    double   latency_ms;  /* arrival -> result in the caller's buffers */
+    double   queue_ms;    /* arrival -> admission to a slot */
+    double   device_ms;   /* hipEvent time of the run */
+} pbbs_server_result;
+
+int pbbs_server_create(int device, const pbbs_server_config* cfg, pbbs_server** out);
+/* Finishes every submitted query, then releases everything. */
+int pbbs_server_destroy(pbbs_server* s);
+/* Uploads a CSR (as pbbs_graph_upload) and returns its id. */
+int pbbs_server_add_graph(pbbs_server* s, int64_t n, int64_t m, const void* offsets,
+                          const int32_t* adj, uint32_t flags, int32_t* graph_id);
+/* Queues a BFS; parent/depth (host, n entries each, either may be NULL) must
+ * stay valid until its result is polled. */
+int pbbs_server_submit(pbbs_server* s, int32_t graph_id, int32_t source, int32_t* parent,
+                       int32_t* depth, uint64_t* ticket);
+/* Takes up to max finished results; waits up to wait_ms (-1 = until one). */
+int pbbs_server_poll(pbbs_server* s, pbbs_server_result* out, int32_t max, int32_t* count,
+                     int32_t wait_ms);
+int pbbs_server_load(pbbs_server* s, int64_t* queued, int64_t* running, int64_t* hbm_in_use);
+
 /* Per-level timing of one BFS, for tuning and measurement: every level kernel
  * is launched on its own with a hipEvent after it (no graph replay), so the
  * times exclude launch gaps.  Writes up to `max_levels` records and the number

@@ -7,6 +7,7 @@ from .core import (  # noqa: F401
     Context,
     DeviceGraph,
     HostGraph,
+    Server,
     from_edges,
     grid3d,
     randlocal,
@@ -18,7 +19,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "BFS", "Context", "DeviceGraph", "HostGraph", "PbbsError", "from_edges", "grid3d",
+    "BFS", "Context", "DeviceGraph", "HostGraph", "PbbsError", "Server", "from_edges", "grid3d",
     "randlocal", "read_graph_from_file", "read_int_seq_from_file", "rmat",
     "write_graph_to_file", "write_sequence_to_file",
 ]

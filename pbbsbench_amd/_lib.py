@@ -85,6 +85,31 @@ class Csr(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("slots", C.c_int32),
+        ("max_batch", C.c_int32),
+        ("hbm_budget", C.c_int64),
+        ("reserved", C.c_int32 * 4),
+    ]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [
+        ("ticket", C.c_uint64),
+        ("graph", C.c_int32),
+        ("source", C.c_int32),
+        ("status", C.c_int32),
+        ("batch", C.c_int32),
+        ("latency_ms", C.c_double),
+        ("queue_ms", C.c_double),
+        ("device_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # (name, restype, argtypes) for every symbol include/pbbs_bfs.h declares.
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
@@ -105,6 +130,16 @@ SIGNATURES = [
     ("pbbs_bfs_profile", C.c_int,
      [_P, _P, C.c_int32, C.POINTER(LevelProfile), C.c_int32, C.POINTER(C.c_int32),
       C.POINTER(C.c_double)]),
+    ("pbbs_server_create", C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    ("pbbs_server_destroy", C.c_int, [_P]),
+    ("pbbs_server_add_graph", C.c_int,
+     [_P, C.c_int64, C.c_int64, _P, _P, C.c_uint32, C.POINTER(C.c_int32)]),
+    ("pbbs_server_submit", C.c_int,
+     [_P, C.c_int32, C.c_int32, _P, _P, C.POINTER(C.c_uint64)]),
+    ("pbbs_server_poll", C.c_int,
+     [_P, C.POINTER(ServerResult), C.c_int32, C.POINTER(C.c_int32), C.c_int32]),
+    ("pbbs_server_load", C.c_int,
+     [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     ("pbbs_csr_free", None, [C.POINTER(Csr)]),
     ("pbbs_gen_rmat", C.c_int,
      [C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,

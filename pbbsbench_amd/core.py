@@ -286,6 +286,107 @@ class DeviceGraph:
         return st.as_dict()
 
 
+class Server:
+    """Serving loop over resident graphs (pbbs_server_* in include/pbbs_bfs.h).
+
+    submit() queues a BFS and returns a ticket; the output arrays are owned by
+    the Server until the ticket's result comes back from poll().  Queries for
+    one symmetric graph that wait together are served by one batched
+    traversal; each result is identical to a lone Context BFS."""
+
+    def __init__(self, device=0, slots=0, max_batch=0, hbm_budget=0):
+        cfg = _lib.ServerConfig()
+        cfg.slots = slots
+        cfg.max_batch = max_batch
+        cfg.hbm_budget = hbm_budget
+        h = C.c_void_p()
+        check(lib().pbbs_server_create(device, C.byref(cfg), C.byref(h)), "server_create")
+        self._h = h
+        self._graph_n = {}
+        self._outputs = {}
+
+    def close(self):
+        if self._h:
+            lib().pbbs_server_destroy(self._h)
+            self._h = None
+            self._outputs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_graph(self, g, wide_offsets=False, assume_symmetric=False):
+        flags = 0
+        if wide_offsets:
+            off = g.offsets
+            flags |= _lib.PBBS_GRAPH_OFFSETS_I64
+        else:
+            if g.m >= 2**32:
+                raise ValueError("m >= 2^32 needs wide offsets")
+            off = g.offsets.astype(np.uint32)
+        if assume_symmetric:
+            flags |= _lib.PBBS_GRAPH_ASSUME_SYMMETRIC
+        gid = C.c_int32()
+        adj = g.adj if g.m > 0 else np.zeros(1, dtype=np.int32)
+        check(lib().pbbs_server_add_graph(self._h, g.n, g.m, _ptr(off), _ptr(adj), flags,
+                                          C.byref(gid)), "server_add_graph")
+        self._graph_n[gid.value] = g.n
+        return gid.value
+
+    def submit(self, graph_id, source, depth=True):
+        """Queues BFS(source) on a graph; returns the ticket."""
+        n = max(self._graph_n.get(graph_id, 1), 1)   # an unknown id is rejected below
+        parent = np.empty(n, dtype=np.int32)
+        dep = np.empty(n, dtype=np.int32) if depth else None
+        t = C.c_uint64()
+        check(lib().pbbs_server_submit(self._h, graph_id, int(source), _ptr(parent),
+                                       _ptr(dep) if depth else None, C.byref(t)),
+              "server_submit")
+        self._outputs[t.value] = (parent, dep)
+        return t.value
+
+    def poll(self, max_results=256, wait_ms=0):
+        """Finished queries: a list of (result dict, parent, depth or None)."""
+        buf = (_lib.ServerResult * max_results)()
+        cnt = C.c_int32()
+        check(lib().pbbs_server_poll(self._h, buf, max_results, C.byref(cnt), wait_ms),
+              "server_poll")
+        out = []
+        for r in buf[:cnt.value]:
+            parent, dep = self._outputs.pop(r.ticket)
+            out.append((r.as_dict(), parent, dep))
+        return out
+
+    def drain(self, tickets, timeout_ms=60000):
+        """Polls until every ticket in `tickets` has finished; returns
+        {ticket: (result, parent, depth)}."""
+        want = set(tickets)
+        got = {}
+        import time
+        end = time.monotonic() + timeout_ms / 1000.0
+        while want:
+            if time.monotonic() > end:
+                raise TimeoutError(f"{len(want)} queries unfinished")
+            for r, p, d in self.poll(wait_ms=100):
+                got[r["ticket"]] = (r, p, d)
+                want.discard(r["ticket"])
+        return got
+
+    def load(self):
+        q, r, h = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().pbbs_server_load(self._h, C.byref(q), C.byref(r), C.byref(h)),
+              "server_load")
+        return {"queued": q.value, "running": r.value, "hbm_in_use": h.value}
+
+
 def BFS(start, G, verbose=False):
     """Parent sequence of a BFS from `start`: -1 = unvisited, the start
     points to itself (bench/BFS.h:9-11)."""

@@ -101,6 +101,16 @@ struct pbbs_ctx {
   BfsState* host_state = nullptr;        // pinned: stats read-back
   hipEvent_t ev_start = nullptr, ev_end = nullptr, ev_c0 = nullptr, ev_c1 = nullptr;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> level_graphs;
+  // the BFS in flight on this context's stream (one at a time)
+  struct Run {
+    int kind = 0;  // 0 single-source, 1 batched
+    bool off64 = false;
+    bool sorted = false;
+    int grid = 0;
+    int K = 0;
+    int64_t launched = 0;
+    int64_t limit = 0;
+  } run;
   // batched BFS
   int64_t cap_bn = 0;                    // vertices the mask buffers hold
   unsigned long long* b_masks = nullptr; // vis | fr0 | fr1 | fr2 | fr3
@@ -110,6 +120,8 @@ struct pbbs_ctx {
   int64_t cap_bout = 0;                  // int32 entries of each output buffer
   int32_t* b_parent = nullptr;
   int32_t* b_depth = nullptr;
+  bool b_last_args_off64 = false;
+  alignas(16) unsigned char b_last_args[sizeof(MbfsArgs<int64_t>)] = {};  // host copy for stats
   std::map<std::tuple<int, int, int>, hipGraphExec_t> batch_graphs;
 };
 
@@ -499,73 +511,136 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   return a;
 }
 
+// --- the level loop as a state machine: begin_* launches the init work and
+// the first chunk of level launches; poll_run checks the chunk's completion
+// event and either reports the BFS done (the device set host_status[0]) or
+// launches the next chunk.  pbbs_bfs drives it to completion; the server
+// drives many contexts at once without blocking.
+
 template <typename OffT>
-static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
-                   pbbs_bfs_stats* stats) {
+static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out);
+
+static int launch_chunk(pbbs_ctx* c) {
+  pbbs_ctx::Run& r = c->run;
+  if (c->tuning.use_graphs) {
+    hipGraphExec_t exec = nullptr;
+    int rc;
+    if (r.kind == 0)
+      rc = r.off64 ? get_level_graph<int64_t>(c, r.sorted, r.grid, r.K, &exec)
+                   : get_level_graph<uint32_t>(c, r.sorted, r.grid, r.K, &exec);
+    else
+      rc = r.off64 ? get_batch_graph<int64_t>(c, r.grid, r.K, &exec)
+                   : get_batch_graph<uint32_t>(c, r.grid, r.K, &exec);
+    if (rc != PBBS_OK) return rc;
+    HIP_TRY(hipGraphLaunch(exec, c->stream));
+  } else {
+    for (int i = 0; i < r.K; ++i) {
+      if (r.kind == 0) {
+        if (r.off64)
+          HIP_TRY(launch_bfs_level<int64_t>(static_cast<const BfsArgs<int64_t>*>(c->w_args),
+                                            r.sorted, r.grid, c->stream));
+        else
+          HIP_TRY(launch_bfs_level<uint32_t>(static_cast<const BfsArgs<uint32_t>*>(c->w_args),
+                                             r.sorted, r.grid, c->stream));
+      } else {
+        if (r.off64)
+          HIP_TRY(launch_mbfs_level<int64_t>(static_cast<const MbfsArgs<int64_t>*>(c->b_args),
+                                             r.grid, c->stream));
+        else
+          HIP_TRY(launch_mbfs_level<uint32_t>(static_cast<const MbfsArgs<uint32_t>*>(c->b_args),
+                                              r.grid, c->stream));
+      }
+    }
+  }
+  r.launched += r.K;
+  HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+  return PBBS_OK;
+}
+
+// Advances the BFS in flight.  With wait, blocks for the current chunk.
+static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
+  *done = false;
+  if (wait) {
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+  } else {
+    const hipError_t e = hipEventQuery(c->ev_end);
+    if (e == hipErrorNotReady) return PBBS_OK;
+    if (e != hipSuccess) return fail(PBBS_ERR_HIP, "event query: %s", hipGetErrorString(e));
+  }
+  if (c->host_status[0]) {
+    *done = true;
+    return PBBS_OK;
+  }
+  pbbs_ctx::Run& r = c->run;
+  if (r.launched > r.limit)
+    return fail(PBBS_ERR_STATE, "BFS did not finish after %lld level launches",
+                (long long)r.launched);
+  r.K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(r.launched, 1 << 20));
+  return launch_chunk(c);
+}
+
+template <typename OffT>
+static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth) {
   const int grid = level_grid(c, g->off64, g->sorted);
-  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
-  if (brc != PBBS_OK) return brc;
+  int rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
+  if (rc != PBBS_OK) return rc;
   BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
+  pbbs_ctx::Run& r = c->run;
+  r.kind = 0;
+  r.off64 = g->off64;
+  r.sorted = g->sorted;
+  r.grid = grid;
+  r.K = c->tuning.chunk_levels;
+  r.launched = 0;
+  r.limit = g->n + 2;
+  if (c->tuning.use_graphs) {  // capture (first use only) while the stream is idle
+    hipGraphExec_t exec = nullptr;
+    rc = get_level_graph<OffT>(c, g->sorted, grid, r.K, &exec);
+    if (rc != PBBS_OK) return rc;
+  }
   volatile int32_t* hs = c->host_status;
   hs[0] = 0;
   hs[1] = 0;
-  int K = c->tuning.chunk_levels;
-  hipGraphExec_t exec = nullptr;
-  // capture (first use only) while the stream is idle
-  if (c->tuning.use_graphs) {
-    const int rc = get_level_graph<OffT>(c, g->sorted, grid, K, &exec);
-    if (rc != PBBS_OK) return rc;
-  }
   HIP_TRY(hipEventRecord(c->ev_start, c->stream));
   HIP_TRY(launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(c->w_args), g->d_iso, s, init_grid,
                                 c->stream));
-  int64_t launched = 0;
-  for (;;) {
-    if (c->tuning.use_graphs) {
-      if (!exec) {
-        const int rc = get_level_graph<OffT>(c, g->sorted, grid, K, &exec);
-        if (rc != PBBS_OK) return rc;
-      }
-      HIP_TRY(hipGraphLaunch(exec, c->stream));
-      exec = nullptr;
-    } else {
-      for (int i = 0; i < K; ++i)
-        HIP_TRY(launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), g->sorted,
-                                       grid, c->stream));
-    }
-    launched += K;
-    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev_end));
-    if (hs[0]) break;
-    if (launched > g->n + 2)
-      return fail(PBBS_ERR_STATE, "BFS did not finish after %lld level launches", (long long)launched);
-    K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(launched, 1 << 20));
-  }
+  return launch_chunk(c);
+}
+
+static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
-  if (stats) {
-    HIP_TRY(hipMemcpyAsync(c->host_state, c->w_state, offsetof(BfsState, log),
-                           hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    std::memset(stats, 0, sizeof *stats);
-    stats->levels = c->host_state->max_depth + 1;
-    stats->td_levels = c->host_state->td_levels;
-    stats->bu_levels = c->host_state->bu_levels;
-    stats->launches = (int32_t)launched;
-    stats->reached = (int64_t)c->host_state->reached;
-    stats->edges_reached = (int64_t)c->host_state->edges_reached;
-    stats->device_ms = ms;
-  }
+  HIP_TRY(hipMemcpyAsync(c->host_state, c->w_state, offsetof(BfsState, log),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::memset(stats, 0, sizeof *stats);
+  stats->levels = c->host_state->max_depth + 1;
+  stats->td_levels = c->host_state->td_levels;
+  stats->bu_levels = c->host_state->bu_levels;
+  stats->launches = (int32_t)c->run.launched;
+  stats->reached = (int64_t)c->host_state->reached;
+  stats->edges_reached = (int64_t)c->host_state->edges_reached;
+  stats->device_ms = ms;
   return PBBS_OK;
+}
+
+template <typename OffT>
+static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
+                   pbbs_bfs_stats* stats) {
+  int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth);
+  bool done = false;
+  while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
+  if (rc != PBBS_OK) return rc;
+  return stats ? single_stats(c, stats) : PBBS_OK;
 }
 
 extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* parent,
                         int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats) {
   clear_error();
   if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
-  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (source < 0 || (int64_t)source >= g->n)
     return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
   HIP_TRY(hipSetDevice(c->device));
@@ -677,7 +752,7 @@ extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
   clear_error();
   if (!c || !g || !count || (max_levels > 0 && !out))
     return fail(PBBS_ERR_INVALID, "NULL argument");
-  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (source < 0 || (int64_t)source >= g->n)
     return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
   HIP_TRY(hipSetDevice(c->device));
@@ -755,8 +830,8 @@ static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out) {
 }
 
 template <typename OffT>
-static int run_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent, int32_t* d_depth,
-                     pbbs_bfs_stats* stats) {
+static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent,
+                       int32_t* d_depth) {
   const int64_t n = g->n;
   MbfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
@@ -782,62 +857,63 @@ static int run_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent
   a.srcmask = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
   a.td_to_bu_div = c->tuning.td_to_bu_div;
   a.bu_to_td_div = c->tuning.bu_to_td_div;
-
-  const int grid = batch_grid(c, g->off64);
+  pbbs_ctx::Run& r = c->run;
+  r.kind = 1;
+  r.off64 = g->off64;
+  r.sorted = g->sorted;
+  r.grid = batch_grid(c, g->off64);
+  r.K = c->tuning.chunk_levels;
+  r.launched = 0;
+  r.limit = n + 2;
   const int setup_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, c->num_cus * 16));
+  if (c->tuning.use_graphs) {
+    hipGraphExec_t exec = nullptr;
+    const int rc = get_batch_graph<OffT>(c, r.grid, r.K, &exec);
+    if (rc != PBBS_OK) return rc;
+  }
   volatile int32_t* hs = c->host_status;
   hs[0] = 0;
   hs[1] = 0;
-  int K = c->tuning.chunk_levels;
-  hipGraphExec_t exec = nullptr;
-  if (c->tuning.use_graphs) {
-    const int rc = get_batch_graph<OffT>(c, grid, K, &exec);
-    if (rc != PBBS_OK) return rc;
-  }
   HIP_TRY(hipEventRecord(c->ev_start, c->stream));
   HIP_TRY(launch_mbfs_setup<OffT>(a, static_cast<MbfsArgs<OffT>*>(c->b_args), setup_grid,
                                   c->stream));
-  int64_t launched = 0;
-  for (;;) {
-    if (c->tuning.use_graphs) {
-      if (!exec) {
-        const int rc = get_batch_graph<OffT>(c, grid, K, &exec);
-        if (rc != PBBS_OK) return rc;
-      }
-      HIP_TRY(hipGraphLaunch(exec, c->stream));
-      exec = nullptr;
-    } else {
-      for (int i = 0; i < K; ++i)
-        HIP_TRY(launch_mbfs_level<OffT>(static_cast<const MbfsArgs<OffT>*>(c->b_args), grid,
-                                        c->stream));
-    }
-    launched += K;
-    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
-    HIP_TRY(hipEventSynchronize(c->ev_end));
-    if (hs[0]) break;
-    if (launched > n + 2)
-      return fail(PBBS_ERR_STATE, "batched BFS did not finish after %lld level launches",
-                  (long long)launched);
-    K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(launched, 1 << 20));
-  }
+  c->b_last_args_off64 = g->off64;
+  std::memcpy(c->b_last_args, &a, sizeof a);
+  return launch_chunk(c);
+}
+
+static int batch_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
-  if (stats) {
-    HIP_TRY(launch_mbfs_stats<OffT>(a, c->num_cus * 4, c->stream));
-    MbfsState hst;
-    HIP_TRY(hipMemcpyAsync(&hst, c->b_state, sizeof hst, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    std::memset(stats, 0, sizeof *stats);
-    stats->levels = hst.max_depth + 1;
-    stats->td_levels = hst.push_levels;
-    stats->bu_levels = hst.pull_levels;
-    stats->launches = (int32_t)launched;
-    stats->reached = (int64_t)hst.reached_pairs;
-    stats->edges_reached = (int64_t)hst.edges_pairs;
-    stats->device_ms = ms;
-  }
+  if (c->b_last_args_off64)
+    HIP_TRY(launch_mbfs_stats<int64_t>(*reinterpret_cast<MbfsArgs<int64_t>*>(c->b_last_args),
+                                       c->num_cus * 4, c->stream));
+  else
+    HIP_TRY(launch_mbfs_stats<uint32_t>(*reinterpret_cast<MbfsArgs<uint32_t>*>(c->b_last_args),
+                                        c->num_cus * 4, c->stream));
+  MbfsState hst;
+  HIP_TRY(hipMemcpyAsync(&hst, c->b_state, sizeof hst, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::memset(stats, 0, sizeof *stats);
+  stats->levels = hst.max_depth + 1;
+  stats->td_levels = hst.push_levels;
+  stats->bu_levels = hst.pull_levels;
+  stats->launches = (int32_t)c->run.launched;
+  stats->reached = (int64_t)hst.reached_pairs;
+  stats->edges_reached = (int64_t)hst.edges_pairs;
+  stats->device_ms = ms;
   return PBBS_OK;
+}
+
+template <typename OffT>
+static int run_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent, int32_t* d_depth,
+                     pbbs_bfs_stats* stats) {
+  int rc = begin_batch<OffT>(c, g, nsrc, d_parent, d_depth);
+  bool done = false;
+  while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
+  if (rc != PBBS_OK) return rc;
+  return stats ? batch_stats(c, stats) : PBBS_OK;
 }
 
 extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
@@ -845,7 +921,7 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
                               pbbs_bfs_stats* stats) {
   clear_error();
   if (!c || !g || !sources) return fail(PBBS_ERR_INVALID, "NULL argument");
-  if (g->ctx != c) return fail(PBBS_ERR_INVALID, "graph belongs to another context");
+  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (nsrc < 1 || nsrc > kMaxSources)
     return fail(PBBS_ERR_INVALID, "source count %d outside [1, %d]", nsrc, kMaxSources);
   for (int i = 0; i < nsrc; ++i)
@@ -887,3 +963,60 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
   }
   return PBBS_OK;
 }
+
+// ------------------------------------------------- entry points for the server
+namespace pbbs {
+
+int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
+  const int64_t nwords = (n + kWave - 1) / kWave;
+  int64_t b = 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * n * 8;
+  if (nsrc > 0) b += 5 * n * 8 + 2 * (int64_t)nsrc * n * 4;
+  return b;
+}
+
+hipStream_t ctx_stream(pbbs_ctx* c) { return c->stream; }
+int64_t graph_n(const pbbs_graph* g) { return g->n; }
+bool graph_symmetric(const pbbs_graph* g) { return g->symmetric; }
+
+// Starts a single-source BFS (nsrc == 1) or a batched one on c's stream,
+// with outputs in c's own buffers.
+int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc) {
+  clear_error();
+  if (!c || !g || !sources || nsrc < 1 || nsrc > kMaxSources)
+    return fail(PBBS_ERR_INVALID, "bad async BFS arguments");
+  for (int i = 0; i < nsrc; ++i)
+    if (sources[i] < 0 || (int64_t)sources[i] >= g->n)
+      return fail(PBBS_ERR_INVALID, "source %d outside the graph", sources[i]);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = ensure_workspace(c, g->n);
+  if (rc != PBBS_OK) return rc;
+  if (nsrc == 1)
+    return g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth)
+                    : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth);
+  if (!g->symmetric) return fail(PBBS_ERR_INVALID, "batched BFS needs a symmetric graph");
+  rc = ensure_batch_workspace(c, g->n, (int64_t)nsrc * g->n, true);
+  if (rc != PBBS_OK) return rc;
+  HIP_TRY(hipMemcpyAsync(c->b_sources, sources, nsrc * sizeof(int32_t), hipMemcpyHostToDevice,
+                         c->stream));
+  return g->off64 ? begin_batch<int64_t>(c, g, nsrc, c->b_parent, c->b_depth)
+                  : begin_batch<uint32_t>(c, g, nsrc, c->b_parent, c->b_depth);
+}
+
+int async_poll(pbbs_ctx* c, bool* done) { return poll_run(c, false, done); }
+
+int async_outputs(pbbs_ctx* c, const int32_t** parent, const int32_t** depth) {
+  if (c->run.kind == 0) {
+    *parent = c->w_parent;
+    *depth = c->w_depth;
+  } else {
+    *parent = c->b_parent;
+    *depth = c->b_depth;
+  }
+  return PBBS_OK;
+}
+
+int async_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
+  return c->run.kind == 0 ? single_stats(c, stats) : batch_stats(c, stats);
+}
+
+}  // namespace pbbs

@@ -1,0 +1,381 @@
+// BFS serving loop: graphs stay resident in HBM; queries arrive on any host
+// thread; a worker thread keeps several runs in flight, one per execution
+// slot (a context of its own: stream, workspace, captured level graphs).
+// Queries are admitted as slots and the HBM budget free up.  When a slot
+// frees and several queries for one graph wait, up to max_batch of them are
+// merged into one batched traversal.  A run's level loop is advanced by
+// polling its chunk event, so runs on different slots overlap on the device;
+// the level kernels never wait on other workgroups, so any mix of concurrent
+// runs completes.  Results are copied into the caller's buffers through
+// pinned staging, and each carries its arrival-to-ready latency.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/pbbs_bfs.h"
+#include "engine.hpp"
+
+using pbbs::fail;
+
+// Internal engine entry points shared with the server (engine.cpp).
+namespace pbbs {
+int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc);
+int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc);
+int async_poll(pbbs_ctx* c, bool* done);
+int async_outputs(pbbs_ctx* c, const int32_t** parent, const int32_t** depth);
+int async_stats(pbbs_ctx* c, pbbs_bfs_stats* stats);
+hipStream_t ctx_stream(pbbs_ctx* c);
+int64_t graph_n(const pbbs_graph* g);
+bool graph_symmetric(const pbbs_graph* g);
+}  // namespace pbbs
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+  uint64_t ticket;
+  int32_t graph;
+  int32_t source;
+  int32_t* parent;
+  int32_t* depth;
+  Clock::time_point arrival;
+  Clock::time_point admitted;
+};
+
+struct Slot {
+  pbbs_ctx* ctx = nullptr;
+  int phase = 0;  // 0 idle, 1 level loop running, 2 outputs being copied
+  std::vector<Request> reqs;
+  int32_t graph = -1;
+  pbbs_graph* g = nullptr;  // captured under the lock: add_graph may grow the table
+  int64_t n = 0;
+  int64_t reserved = 0;  // HBM bytes charged to this slot's run
+  int32_t* pin_parent = nullptr;
+  int32_t* pin_depth = nullptr;
+  int64_t pin_cap = 0;
+  hipEvent_t copied = nullptr;
+  double device_ms = 0;
+};
+
+}  // namespace
+
+struct pbbs_server {
+  int device = 0;
+  pbbs_ctx* main = nullptr;
+  std::vector<pbbs_graph*> graphs;
+  std::vector<Slot> slots;
+  int32_t max_batch = 64;
+  int64_t budget = 0;
+  int64_t in_use = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request> pending;
+  std::vector<pbbs_server_result> completed;
+  int64_t running = 0;
+  uint64_t next_ticket = 1;
+  bool stop = false;
+  std::thread worker;
+};
+
+namespace {
+
+void finish(pbbs_server* s, Slot& sl, int status) {
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    const Request& r = sl.reqs[i];
+    pbbs_server_result res{};
+    res.ticket = r.ticket;
+    res.graph = r.graph;
+    res.source = r.source;
+    res.status = status;
+    res.batch = (int32_t)sl.reqs.size();
+    res.latency_ms = ms_between(r.arrival, now);
+    res.queue_ms = ms_between(r.arrival, r.admitted);
+    res.device_ms = sl.device_ms;
+    s->completed.push_back(res);
+  }
+  s->running -= (int64_t)sl.reqs.size();
+  s->in_use -= sl.reserved;
+  sl.reqs.clear();
+  sl.reserved = 0;
+  sl.phase = 0;
+  s->cv_done.notify_all();
+}
+
+int ensure_pinned(Slot& sl, int64_t entries) {
+  if (entries <= sl.pin_cap) return PBBS_OK;
+  if (sl.pin_parent) (void)hipHostFree(sl.pin_parent);
+  if (sl.pin_depth) (void)hipHostFree(sl.pin_depth);
+  sl.pin_parent = sl.pin_depth = nullptr;
+  sl.pin_cap = 0;
+  if (hipHostMalloc((void**)&sl.pin_parent, entries * sizeof(int32_t), hipHostMallocDefault) !=
+          hipSuccess ||
+      hipHostMalloc((void**)&sl.pin_depth, entries * sizeof(int32_t), hipHostMallocDefault) !=
+          hipSuccess)
+    return fail(PBBS_ERR_NOMEM, "pinned staging of %lld entries failed", (long long)entries);
+  sl.pin_cap = entries;
+  return PBBS_OK;
+}
+
+// Takes the oldest waiting query and, for a symmetric graph, up to
+// max_batch - 1 more for the same graph, if the HBM budget allows the run.
+bool admit(pbbs_server* s, Slot& sl) {
+  std::vector<Request> take;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->pending.empty()) return false;
+    const int32_t gid = s->pending.front().graph;
+    pbbs_graph* g = s->graphs[gid];
+    const int64_t n = pbbs::graph_n(g);
+    const int cap = pbbs::graph_symmetric(g) ? s->max_batch : 1;
+    int want = 0;
+    for (const Request& r : s->pending)
+      if (r.graph == gid && want < cap) ++want;
+    // Shrink the batch to what the budget holds now; a lone query always fits
+    // an empty server (submit rejects graphs that do not).
+    int64_t need = pbbs::bfs_workspace_bytes(n, want > 1 ? want : 0);
+    while (want > 1 && s->in_use + need > s->budget) {
+      --want;
+      need = pbbs::bfs_workspace_bytes(n, want > 1 ? want : 0);
+    }
+    if (s->in_use + need > s->budget && s->in_use > 0) return false;  // wait for memory
+    for (auto it = s->pending.begin(); it != s->pending.end() && (int)take.size() < want;) {
+      if (it->graph == gid) {
+        take.push_back(*it);
+        it = s->pending.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    s->in_use += need;
+    s->running += (int64_t)take.size();
+    sl.reserved = need;
+    sl.graph = gid;
+    sl.g = g;
+    sl.n = n;
+  }
+  const auto now = Clock::now();
+  for (Request& r : take) r.admitted = now;
+  sl.reqs = std::move(take);
+  std::vector<int32_t> src(sl.reqs.size());
+  for (size_t i = 0; i < src.size(); ++i) src[i] = sl.reqs[i].source;
+  int rc = ensure_pinned(sl, (int64_t)src.size() * sl.n);
+  if (rc == PBBS_OK)
+    rc = pbbs::async_begin(sl.ctx, sl.g, src.data(), (int32_t)src.size());
+  if (rc != PBBS_OK) {
+    finish(s, sl, rc);
+    return true;
+  }
+  sl.phase = 1;
+  return true;
+}
+
+// Level loop finished: stats, then copy outputs to pinned staging.
+int start_copy(Slot& sl) {
+  pbbs_bfs_stats st;
+  int rc = pbbs::async_stats(sl.ctx, &st);
+  if (rc != PBBS_OK) return rc;
+  sl.device_ms = st.device_ms;
+  const int32_t *dp = nullptr, *dd = nullptr;
+  rc = pbbs::async_outputs(sl.ctx, &dp, &dd);
+  if (rc != PBBS_OK) return rc;
+  const int64_t bytes = (int64_t)sl.reqs.size() * sl.n * sizeof(int32_t);
+  hipStream_t st_ = pbbs::ctx_stream(sl.ctx);
+  if (hipMemcpyAsync(sl.pin_parent, dp, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess ||
+      hipMemcpyAsync(sl.pin_depth, dd, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess ||
+      hipEventRecord(sl.copied, st_) != hipSuccess)
+    return fail(PBBS_ERR_HIP, "output copy failed");
+  sl.phase = 2;
+  return PBBS_OK;
+}
+
+void deliver(Slot& sl) {
+  for (size_t i = 0; i < sl.reqs.size(); ++i) {
+    const Request& r = sl.reqs[i];
+    if (r.parent) std::memcpy(r.parent, sl.pin_parent + i * sl.n, sl.n * sizeof(int32_t));
+    if (r.depth) std::memcpy(r.depth, sl.pin_depth + i * sl.n, sl.n * sizeof(int32_t));
+  }
+}
+
+void worker_loop(pbbs_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    bool active = false;
+    for (Slot& sl : s->slots) {
+      if (sl.phase == 1) {
+        bool done = false;
+        int rc = pbbs::async_poll(sl.ctx, &done);
+        if (rc == PBBS_OK && done) rc = start_copy(sl);
+        if (rc != PBBS_OK) finish(s, sl, rc);
+      } else if (sl.phase == 2) {
+        const hipError_t e = hipEventQuery(sl.copied);
+        if (e == hipSuccess) {
+          deliver(sl);
+          finish(s, sl, PBBS_OK);
+        } else if (e != hipErrorNotReady) {
+          finish(s, sl, PBBS_ERR_HIP);
+        }
+      }
+    }
+    for (Slot& sl : s->slots)
+      if (sl.phase == 0) admit(s, sl);
+    for (Slot& sl : s->slots) active |= sl.phase != 0;
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stop && !active && s->pending.empty()) break;
+    if (!active && s->pending.empty()) {
+      s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+    } else {
+      lk.unlock();
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbbs_server_create(int device, const pbbs_server_config* cfg, pbbs_server** out) {
+  pbbs::clear_error();
+  if (!out) return fail(PBBS_ERR_INVALID, "out is NULL");
+  *out = nullptr;
+  pbbs_server_config c{};
+  if (cfg) c = *cfg;
+  if (c.slots <= 0) c.slots = 4;
+  if (c.max_batch <= 0) c.max_batch = PBBS_MAX_BATCH;
+  if (c.slots > 64 || c.max_batch > PBBS_MAX_BATCH)
+    return fail(PBBS_ERR_INVALID, "slots <= 64 and max_batch <= %d", PBBS_MAX_BATCH);
+  pbbs_server* s = new pbbs_server();
+  s->device = device;
+  s->max_batch = c.max_batch;
+  int rc = pbbs_ctx_create(device, &s->main);
+  if (rc != PBBS_OK) {
+    delete s;
+    return rc;
+  }
+  if (c.hbm_budget > 0) {
+    s->budget = c.hbm_budget;
+  } else {
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    s->budget = (int64_t)(free_b * 0.8);
+  }
+  s->slots.resize(c.slots);
+  for (Slot& sl : s->slots) {
+    rc = pbbs_ctx_create(device, &sl.ctx);
+    if (rc == PBBS_OK && hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming) != hipSuccess)
+      rc = fail(PBBS_ERR_HIP, "event creation failed");
+    if (rc != PBBS_OK) {
+      pbbs_server_destroy(s);
+      return rc;
+    }
+  }
+  s->worker = std::thread(worker_loop, s);
+  *out = s;
+  return PBBS_OK;
+}
+
+int pbbs_server_destroy(pbbs_server* s) {
+  if (!s) return PBBS_OK;
+  if (s->worker.joinable()) {
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->worker.join();
+  }
+  for (Slot& sl : s->slots) {
+    if (sl.copied) (void)hipEventDestroy(sl.copied);
+    if (sl.pin_parent) (void)hipHostFree(sl.pin_parent);
+    if (sl.pin_depth) (void)hipHostFree(sl.pin_depth);
+    pbbs_ctx_destroy(sl.ctx);
+  }
+  for (pbbs_graph* g : s->graphs) pbbs_graph_free(g);
+  pbbs_ctx_destroy(s->main);
+  delete s;
+  return PBBS_OK;
+}
+
+int pbbs_server_add_graph(pbbs_server* s, int64_t n, int64_t m, const void* offsets,
+                          const int32_t* adj, uint32_t flags, int32_t* graph_id) {
+  pbbs::clear_error();
+  if (!s || !graph_id) return fail(PBBS_ERR_INVALID, "NULL argument");
+  pbbs_graph* g = nullptr;
+  const int rc = pbbs_graph_upload(s->main, n, m, offsets, adj, flags, &g);
+  if (rc != PBBS_OK) return rc;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->graphs.push_back(g);
+  *graph_id = (int32_t)s->graphs.size() - 1;
+  return PBBS_OK;
+}
+
+int pbbs_server_submit(pbbs_server* s, int32_t graph_id, int32_t source, int32_t* parent,
+                       int32_t* depth, uint64_t* ticket) {
+  pbbs::clear_error();
+  if (!s) return fail(PBBS_ERR_INVALID, "NULL server");
+  Request r{};
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (graph_id < 0 || graph_id >= (int32_t)s->graphs.size())
+      return fail(PBBS_ERR_INVALID, "unknown graph %d", graph_id);
+    if (source < 0 || (int64_t)source >= pbbs::graph_n(s->graphs[graph_id]))
+      return fail(PBBS_ERR_INVALID, "source %d outside graph %d", source, graph_id);
+    if (pbbs::bfs_workspace_bytes(pbbs::graph_n(s->graphs[graph_id]), 0) > s->budget)
+      return fail(PBBS_ERR_LIMIT, "graph %d needs more HBM than the server budget", graph_id);
+    r.ticket = s->next_ticket++;
+    r.graph = graph_id;
+    r.source = source;
+    r.parent = parent;
+    r.depth = depth;
+    r.arrival = Clock::now();
+    s->pending.push_back(r);
+  }
+  s->cv_work.notify_one();
+  if (ticket) *ticket = r.ticket;
+  return PBBS_OK;
+}
+
+int pbbs_server_poll(pbbs_server* s, pbbs_server_result* out, int32_t max, int32_t* count,
+                     int32_t wait_ms) {
+  pbbs::clear_error();
+  if (!s || !count || (max > 0 && !out)) return fail(PBBS_ERR_INVALID, "NULL argument");
+  std::unique_lock<std::mutex> lk(s->mu);
+  if (s->completed.empty() && wait_ms != 0) {
+    auto pred = [&] { return !s->completed.empty(); };
+    if (wait_ms < 0)
+      s->cv_done.wait(lk, pred);
+    else
+      s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), pred);
+  }
+  const int32_t k = (int32_t)std::min<size_t>((size_t)std::max(max, 0), s->completed.size());
+  for (int32_t i = 0; i < k; ++i) out[i] = s->completed[i];
+  s->completed.erase(s->completed.begin(), s->completed.begin() + k);
+  *count = k;
+  return PBBS_OK;
+}
+
+int pbbs_server_load(pbbs_server* s, int64_t* queued, int64_t* running, int64_t* hbm_in_use) {
+  pbbs::clear_error();
+  if (!s) return fail(PBBS_ERR_INVALID, "NULL server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (queued) *queued = (int64_t)s->pending.size();
+  if (running) *running = s->running;
+  if (hbm_in_use) *hbm_in_use = s->in_use;
+  return PBBS_OK;
+}
+
+}  // extern "C"
