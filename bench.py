@@ -85,6 +85,12 @@ def main():
                     help="budget of the CPU-baseline sample (0 = skip)")
     ap.add_argument("--td-to-bu-div", type=int, default=None)
     ap.add_argument("--bu-to-td-div", type=int, default=None)
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve"],
+                    help="single: one-source BFS (default); batch: 64 sources in one traversal "
+                         "vs 64 sequential runs on RMAT 2^22; serve: Poisson query stream over "
+                         "8 resident RMAT graphs")
+    ap.add_argument("--qps", type=float, default=2000.0, help="serve: arrival rate")
+    ap.add_argument("--queries", type=int, default=4000, help="serve: queries in the stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
@@ -92,6 +98,10 @@ def main():
               "which adds no figure beyond the one-GPU value.", file=sys.stderr)
         sys.exit(2)
 
+    if args.mode == "batch":
+        return bench_batch(args)
+    if args.mode == "serve":
+        return bench_serve(args)
     import pbbsbench_amd as pb
 
     build, wide, desc = WORKLOADS[args.workload]
@@ -203,6 +213,214 @@ def main():
     dg.free()
     ctx.close()
     print(json.dumps(result))
+
+
+def bench_batch(args):
+    """64 sources in one batched traversal vs 64 sequential single-source runs
+    on RMAT 2^22 (2^26 draws, a=.57 b=.19 c=.19), uint32 offsets.  A step is
+    one batched BFS of all 64 sources with outputs [64][n] written to HBM."""
+    import pbbsbench_amd as pb
+    t0 = time.time()
+    g = pb.rmat(22, 2 ** 26, 0.57, 0.19, 0.19, seed=1)
+    gen_s = time.time() - t0
+    deg = g.degrees()
+    rng = np.random.default_rng(args.seed)
+    src = rng.choice(np.flatnonzero(deg > 0), 64, replace=False).astype(np.int32)
+    ctx = pb.Context(0)
+    dg = ctx.upload(g)
+    for _ in range(max(args.warmup, 1)):
+        dg.bfs_batch_device(src)
+        dg.bfs_device(int(src[0]))
+    # sequential: 64 single-source runs, outputs in HBM
+    seq_ms, seq_edges = 0.0, 0
+    t0 = time.perf_counter()
+    for s in src:
+        st = dg.bfs_device(int(s))
+        seq_ms += st["device_ms"]
+        seq_edges += st["edges_reached"]
+    seq_wall = (time.perf_counter() - t0) * 1e3
+    # batched
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = dg.bfs_batch_device(src)
+        dev_ms.append(st["device_ms"])
+    wall = (time.perf_counter() - t0) / args.steps * 1e3
+    pairs_edges = st["edges_reached"]
+    assert pairs_edges == seq_edges, (pairs_edges, seq_edges)
+    # parity: every batched row equals its sequential run, bit for bit
+    P, D, _ = dg.bfs_batch(src)
+    exact = True
+    for i, s in enumerate(src):
+        p, d, _ = dg.bfs(int(s))
+        exact &= bool(np.array_equal(P[i], p)) and bool(np.array_equal(D[i], d))
+    reached_any = (P != -1).any(axis=0)
+    m_union = int(deg[reached_any].sum())
+    n = g.n
+    B = 4 * (n + 1) + 4 * m_union + 8 * 64 * n
+    ev = float(np.mean(dev_ms))
+    result = {
+        "metric": "batched-64-source aggregate GTEPS (vs 64 sequential runs)",
+        "value": round(pairs_edges / (wall / 1e3) / 1e9, 2),
+        "unit": "GTEPS",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded generator in libpbbs_bfs)",
+        "config": {
+            "workload": "rmat22x64",
+            "description": "RMAT 2^22 vertices, 2^26 draws, a=.57 b=.19 c=.19, symmetrised, "
+                           "uint32 offsets; 64 seeded non-isolated sources",
+            "n": n, "m": g.m, "sources_seed": args.seed,
+            "pairs_reached": st["reached"], "sum_m_r": pairs_edges,
+            "levels": st["levels"], "push_levels": st["td_levels"],
+            "pull_levels": st["bu_levels"],
+            "event_ms_per_batch": round(ev, 3),
+            "sequential_event_ms_64": round(seq_ms, 3),
+            "sequential_wall_ms_64": round(seq_wall, 3),
+            "sequential_gteps": round(seq_edges / (seq_wall / 1e3) / 1e9, 2),
+            "batch_vs_sequential": round(seq_wall / wall, 3),
+            "gen_s": round(gen_s, 1), "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "one batched BFS launch sequence (mbfs clear/seed + mbfs_level_kernel per level)",
+            "achieved": round(B / (ev / 1e3) / 1e9, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(B / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes": int(B),
+        },
+        "parity": "bit-exact vs sequential runs" if exact else "MISMATCH",
+    }
+    if args.cpu_seconds > 0:
+        import oracle  # the checker and the CPU baseline; never the measured product
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        t1 = time.time()
+        done, edges, secs = 0, 0, 0.0
+        for s in src:
+            t2 = time.perf_counter()
+            ref, _ = oracle.det_bfs(g.offsets, g.adj, int(s), threads=threads)
+            secs += time.perf_counter() - t2
+            edges += int(deg[ref != -1].sum())
+            exact &= bool(np.array_equal(ref, P[done]))
+            done += 1
+            if time.time() - t1 > args.cpu_seconds:
+                break
+        result["cpu_baseline"] = {
+            "value": round(edges / secs / 1e9, 3), "unit": "GTEPS", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle det_bfs from the first {done} of the 64 sources, one after another "
+                      f"({secs:.2f} s); CPU: {cpu_info()}",
+        }
+        result["parity"] = ("bit-exact vs sequential runs and vs oracle on the CPU sample"
+                            if exact else "MISMATCH")
+    dg.free()
+    ctx.close()
+    print(json.dumps(result))
+    if not exact:
+        sys.exit(1)
+
+
+SERVE_GRAPHS = [(16, 1), (17, 1), (18, 1), (19, 1), (20, 1), (21, 1), (22, 1), (22, 2)]
+
+
+def bench_serve(args):
+    """Poisson query stream over 8 resident RMAT graphs (scales 16..22 plus a
+    second 2^22 graph; 16 draws per vertex, a=.57 b=.19 c=.19).  Each query
+    picks a graph and a non-isolated source at random; latency is arrival ->
+    parents in the caller's host buffer.  A seeded tenth of the queries is
+    re-run alone afterwards and must match bit for bit."""
+    import threading
+    import zlib
+
+    import pbbsbench_amd as pb
+    rng = np.random.default_rng(args.seed)
+    graphs, nz = [], []
+    t0 = time.time()
+    for scale, seed in SERVE_GRAPHS:
+        g = pb.rmat(scale, 16 * 2 ** scale, 0.57, 0.19, 0.19, seed=seed)
+        graphs.append(g)
+        nz.append(np.flatnonzero(g.degrees() > 0))
+    gen_s = time.time() - t0
+    Q = args.queries
+    gsel = rng.integers(0, len(graphs), Q)
+    srcs = [int(nz[gi][rng.integers(0, nz[gi].size)]) for gi in gsel]
+    gaps = rng.exponential(1.0 / args.qps, Q)
+    check = set(rng.choice(Q, max(1, Q // 10), replace=False).tolist())
+
+    srv = pb.Server(0)
+    ids = [srv.add_graph(g) for g in graphs]
+    # warm-up: one query per graph (captures level graphs, sizes workspaces)
+    srv.drain([srv.submit(ids[i], int(nz[i][0]), depth=False) for i in range(len(graphs))])
+    tickets = {}
+    lock = threading.Lock()
+
+    def client():
+        t = time.perf_counter()
+        for q in range(Q):
+            t += gaps[q]
+            d = t - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            with lock:  # the result may come back before the ticket is recorded otherwise
+                tickets[srv.submit(ids[gsel[q]], srcs[q], depth=False)] = q
+
+    results = {}
+    t_start = time.perf_counter()
+    th = threading.Thread(target=client)
+    th.start()
+    while len(results) < Q:
+        for r, p, _ in srv.poll(wait_ms=50):
+            with lock:
+                q = tickets[r["ticket"]]
+            results[q] = (r, zlib.crc32(p.tobytes()) if q in check else None)
+    th.join()
+    t_total = time.perf_counter() - t_start
+    srv.close()
+    lat = np.array([results[q][0]["latency_ms"] for q in range(Q)])
+    qms = np.array([results[q][0]["queue_ms"] for q in range(Q)])
+    batch = np.array([results[q][0]["batch"] for q in range(Q)])
+    ok = all(results[q][0]["status"] == 0 for q in range(Q))
+    # solo re-runs of the checked queries
+    ctx = pb.Context(0)
+    exact = ok
+    for gi in range(len(graphs)):
+        qs = [q for q in check if gsel[q] == gi]
+        if not qs:
+            continue
+        dg = ctx.upload(graphs[gi])
+        for q in qs:
+            p, _, _ = dg.bfs(srcs[q], depth=False)
+            exact &= zlib.crc32(p.tobytes()) == results[q][1]
+        dg.free()
+    ctx.close()
+    result = {
+        "metric": "BFS serving: p50/p99 latency (arrival -> parents in host memory)",
+        "value": round(float(np.percentile(lat, 99)), 3),
+        "unit": "ms (p99)",
+        "n_gpus": 1, "steps": Q, "warmup": len(graphs),
+        "ms_per_step": round(t_total / Q * 1e3, 4),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int32", "data": "synthetic (seeded generator in libpbbs_bfs)",
+        "config": {
+            "workload": "serve8",
+            "graphs": [f"rmat{s} seed {sd} n={g.n} m={g.m}"
+                       for (s, sd), g in zip(SERVE_GRAPHS, graphs)],
+            "qps_offered": args.qps, "queries": Q,
+            "throughput_qps": round(Q / t_total, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p90_ms": round(float(np.percentile(lat, 90)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "queue_p50_ms": round(float(np.percentile(qms, 50)), 3),
+            "mean_batch": round(float(batch.mean()), 2),
+            "checked_solo": len(check), "gen_s": round(gen_s, 1),
+            "parallelism": "single GPU",
+        },
+        "parity": "bit-exact vs solo runs (checked sample)" if exact else "MISMATCH",
+    }
+    print(json.dumps(result))
+    if not exact:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -77,7 +77,9 @@ typedef struct {
     int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16 */
     int32_t use_graphs;     /* 1 = replay captured HIP graphs of level launches (default),
                                0 = launch each level kernel on the stream */
-    int32_t reserved[2];
+    int32_t batch_pull_div; /* batched BFS: pull a level when |F| + sum deg(F) > m / this,
+                               else push; default 10; 0 = push only */
+    int32_t reserved;
 } pbbs_tuning;
 
 int pbbs_tuning_get(const pbbs_ctx* ctx, pbbs_tuning* out);

@@ -32,7 +32,8 @@ class Tuning(C.Structure):
         ("bu_serial_depth", C.c_int32),
         ("chunk_levels", C.c_int32),
         ("use_graphs", C.c_int32),
-        ("reserved", C.c_int32 * 2),
+        ("batch_pull_div", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

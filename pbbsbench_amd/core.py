@@ -18,6 +18,8 @@ writeSequenceToFile (common/sequenceIO.h:189-193).  Here:
 Every call goes through libpbbs_bfs.so; nothing here computes a BFS on the CPU.
 """
 import ctypes as C
+import threading
+import time
 
 import numpy as np
 
@@ -304,6 +306,7 @@ class Server:
         self._h = h
         self._graph_n = {}
         self._outputs = {}
+        self._lock = threading.Lock()   # a result can be polled before submit() returns
 
     def close(self):
         if self._h:
@@ -347,10 +350,11 @@ class Server:
         parent = np.empty(n, dtype=np.int32)
         dep = np.empty(n, dtype=np.int32) if depth else None
         t = C.c_uint64()
-        check(lib().pbbs_server_submit(self._h, graph_id, int(source), _ptr(parent),
-                                       _ptr(dep) if depth else None, C.byref(t)),
-              "server_submit")
-        self._outputs[t.value] = (parent, dep)
+        with self._lock:
+            check(lib().pbbs_server_submit(self._h, graph_id, int(source), _ptr(parent),
+                                           _ptr(dep) if depth else None, C.byref(t)),
+                  "server_submit")
+            self._outputs[t.value] = (parent, dep)
         return t.value
 
     def poll(self, max_results=256, wait_ms=0):
@@ -360,9 +364,10 @@ class Server:
         check(lib().pbbs_server_poll(self._h, buf, max_results, C.byref(cnt), wait_ms),
               "server_poll")
         out = []
-        for r in buf[:cnt.value]:
-            parent, dep = self._outputs.pop(r.ticket)
-            out.append((r.as_dict(), parent, dep))
+        with self._lock:
+            for r in buf[:cnt.value]:
+                parent, dep = self._outputs.pop(r.ticket)
+                out.append((r.as_dict(), parent, dep))
         return out
 
     def drain(self, tickets, timeout_ms=60000):
@@ -370,7 +375,6 @@ class Server:
         {ticket: (result, parent, depth)}."""
         want = set(tickets)
         got = {}
-        import time
         end = time.monotonic() + timeout_ms / 1000.0
         while want:
             if time.monotonic() > end:

@@ -95,24 +95,29 @@ template <typename OffT>
 struct MbfsArgs {
   const OffT* off;
   const int32_t* adj;
+  const int32_t* adj_rank;     // [m] rank of each adjacency entry (masks are stored by rank)
+  const int32_t* rank;         // [n] rank of v in decreasing-degree order
+  const int32_t* inv_rank;     // [n] vertex of each rank
   int32_t* parent;             // out: [nsrc][n], source-major
   int32_t* depth;              // out: [nsrc][n]
-  unsigned long long* vis;     // [n] sources that reached v
-  unsigned long long* fr[4];   // [n] each
+  unsigned long long* vis;     // [n] sources that reached v (by vertex id)
+  unsigned long long* fr[4];   // [n] each, indexed by rank
+  unsigned long long* fbm[4];  // [nwords] each: bit r set iff fr[.][r] != 0
   int32_t* q[2];               // union queue, kSubQueues regions of qcap
   uint32_t* qe[2];
   MbfsState* st;
   uint32_t* wg_level;
   int32_t* host_status;
   const int32_t* sources;      // [nsrc]
+  const int32_t* pull_rows;    // first row of each wavefront's edge slice (waves + 1 entries)
   int64_t n;
   int64_t m;
   int64_t nwords;
   int64_t qcap;
   unsigned long long srcmask;  // bits of the sources in use
   int32_t nsrc;
-  int32_t td_to_bu_div;
-  int32_t bu_to_td_div;
+  int32_t td_to_bu_div;        // 0 = push levels only
+  int32_t pull_div;            // pull when (|F| + edges(F)) * pull_div > m
   int32_t pad;
 };
 

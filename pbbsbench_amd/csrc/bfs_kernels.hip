@@ -32,6 +32,24 @@
 
 namespace pbbs {
 
+// Device pointers typed in the global address space, so loads, stores and
+// atomics through them compile to global_* instructions.  Generic (flat)
+// accesses also count against the LDS counter, so every LDS or cross-lane
+// wait would otherwise wait for the outstanding memory loads too.
+#define PBBS_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+using gptr = PBBS_GLOBAL T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> glob(T* p) {
+  return (gptr<T>)p;
+}
+// Generic pointer for the atomic builtins; the compiler still sees the
+// global origin and emits global atomics.
+template <typename T>
+__device__ __forceinline__ T* gen(gptr<T> p) {
+  return (T*)p;
+}
+
 #ifdef PBBS_WAVE_TIMING
 // Diagnostic build only: per-wavefront wall-clock stamps (start, end of the
 // first pass, end) of the first 16 levels, for 16384 wavefronts.
@@ -85,7 +103,7 @@ __device__ __forceinline__ int64_t shfl_off(int64_t x, int src) {
   return (int64_t)__shfl((long long)x, src);
 }
 
-__device__ __forceinline__ bool in_set(const unsigned* bits, int u) {
+__device__ __forceinline__ bool in_set(gptr<const unsigned> bits, int u) {
   return (bits[u >> 5] >> (u & 31)) & 1u;
 }
 
@@ -101,21 +119,21 @@ __device__ __forceinline__ long long num_waves() {
 // that no array of the argument block is indexed dynamically.
 template <typename OffT>
 struct LevelView {
-  const OffT* __restrict__ off;
-  const int32_t* __restrict__ adj;
-  int32_t* parent;
-  int32_t* depth;
-  unsigned long long* visited;
-  const unsigned* fb_cur;           // F_k
-  unsigned long long* fb_next;      // F_{k+1}
-  const int32_t* q_cur;             // sub-queue regions of F_k
-  const uint32_t* qe_cur;
-  int32_t* q_next;                  // this workgroup's sub-queue region of F_{k+1}
-  uint32_t* qe_next;
-  unsigned long long* ctr_next;     // its counter
-  uint32_t* d0_next;
-  const unsigned long long* ctr_cur;  // F_k's counters (all sub-queues)
-  const uint32_t* bu_bounds;        // weighted wavefront shares, or null for equal shares
+  gptr<const OffT> off;
+  gptr<const int32_t> adj;
+  gptr<int32_t> parent;
+  gptr<int32_t> depth;
+  gptr<unsigned long long> visited;
+  gptr<const unsigned> fb_cur;          // F_k
+  gptr<unsigned long long> fb_next;     // F_{k+1}
+  gptr<const int32_t> q_cur;            // sub-queue regions of F_k
+  gptr<const uint32_t> qe_cur;
+  gptr<int32_t> q_next;                 // this workgroup's sub-queue region of F_{k+1}
+  gptr<uint32_t> qe_next;
+  gptr<unsigned long long> ctr_next;    // its counter
+  gptr<uint32_t> d0_next;
+  gptr<const unsigned long long> ctr_cur;  // F_k's counters (all sub-queues)
+  gptr<const uint32_t> bu_bounds;       // weighted wavefront shares, or null for equal shares
   long long n;
   long long nwords;
   long long qcap;
@@ -124,14 +142,21 @@ struct LevelView {
 };
 
 // --------------------------------------------------------- frontier staging
-// Per-wavefront LDS buffer of (vertex, degree offset within the buffer).
-struct StageBuf {
-  int32_t v[kWavesPerBlock][kWaveBuf];
-  uint32_t eo[kWavesPerBlock][kWaveBuf];
+// Per-wavefront LDS buffer of (vertex, degree offset within the buffer),
+// kCap entries per wavefront.
+template <int kCap>
+struct StageBufT {
+  static constexpr int kCapacity = kCap;
+  int32_t v[kWavesPerBlock][kCap];
+  uint32_t eo[kWavesPerBlock][kCap];
   uint32_t cnt[kWavesPerBlock];
   uint32_t deg[kWavesPerBlock];
   unsigned long long base;
 };
+using StageBuf = StageBufT<kWaveBuf>;
+// The batched kernel keeps a smaller buffer: its LDS budget sets how many
+// workgroups share a CU, and its appends are few per wavefront.
+using BatchStageBuf = StageBufT<128>;
 
 struct Stage {
   unsigned cnt = 0;   // wave-uniform
@@ -140,8 +165,8 @@ struct Stage {
 
 // Writes buffer entries [0, cnt) of wavefront w to the next frontier at
 // (qbase, ebase) and empties it.
-template <typename View>
-__device__ __forceinline__ void stage_copy_out(const View& a, StageBuf& sb, int w,
+template <typename View, typename SB>
+__device__ __forceinline__ void stage_copy_out(const View& a, SB& sb, int w,
                                                unsigned cnt, unsigned qbase, unsigned ebase) {
   for (unsigned i = lane_id(); i < cnt; i += kWave) {
     a.q_next[qbase + i] = sb.v[w][i];
@@ -149,12 +174,12 @@ __device__ __forceinline__ void stage_copy_out(const View& a, StageBuf& sb, int 
   }
 }
 
-template <typename View>
-__device__ __forceinline__ void stage_flush_wave(const View& a, StageBuf& sb, Stage& s) {
+template <typename View, typename SB>
+__device__ __forceinline__ void stage_flush_wave(const View& a, SB& sb, Stage& s) {
   const int w = threadIdx.x / kWave;
   unsigned long long base = 0;
   if (lane_id() == 0)
-    base = atomicAdd(a.ctr_next, ((unsigned long long)s.deg << 32) | s.cnt);
+    base = atomicAdd(gen(a.ctr_next), ((unsigned long long)s.deg << 32) | s.cnt);
   base = __shfl(base, 0);
   wave_lds_sync();
   stage_copy_out(a, sb, w, s.cnt, (unsigned)base, (unsigned)(base >> 32));
@@ -164,15 +189,15 @@ __device__ __forceinline__ void stage_flush_wave(const View& a, StageBuf& sb, St
 }
 
 // Stages the lanes with `take` (vertex v of degree deg) for F_{k+1}.
-template <typename View>
-__device__ __forceinline__ void stage_push(const View& a, StageBuf& sb, Stage& s, bool take, int v,
+template <typename View, typename SB>
+__device__ __forceinline__ void stage_push(const View& a, SB& sb, Stage& s, bool take, int v,
                                            uint32_t deg) {
   const unsigned long long mask = __ballot(take);
   if (mask == 0) return;
   const unsigned c = (unsigned)__popcll(mask);
   const unsigned incl = wave_inclusive_sum(take ? deg : 0u);
   const unsigned total = __shfl(incl, kWave - 1);
-  if (s.cnt + c > (unsigned)kWaveBuf) stage_flush_wave(a, sb, s);
+  if (s.cnt + c > (unsigned)SB::kCapacity) stage_flush_wave(a, sb, s);
   if (take) {
     const int w = threadIdx.x / kWave;
     const unsigned pos = s.cnt + lanes_below(mask);
@@ -184,8 +209,8 @@ __device__ __forceinline__ void stage_push(const View& a, StageBuf& sb, Stage& s
 }
 
 // End of level: one reservation for the whole workgroup, then copy-out.
-template <typename View>
-__device__ __forceinline__ void stage_flush_block(const View& a, StageBuf& sb, const Stage& s) {
+template <typename View, typename SB>
+__device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const Stage& s) {
   const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
     sb.cnt[w] = s.cnt;
@@ -198,7 +223,7 @@ __device__ __forceinline__ void stage_flush_block(const View& a, StageBuf& sb, c
       tc += sb.cnt[i];
       td += sb.deg[i];
     }
-    sb.base = tc ? atomicAdd(a.ctr_next, ((unsigned long long)td << 32) | tc) : 0ull;
+    sb.base = tc ? atomicAdd(gen(a.ctr_next), ((unsigned long long)td << 32) | tc) : 0ull;
   }
   __syncthreads();
   unsigned qb = (unsigned)sb.base, eb = (unsigned)(sb.base >> 32);
@@ -216,7 +241,7 @@ __device__ __forceinline__ void stage_flush_block(const View& a, StageBuf& sb, c
 // source by a register binary search over the window's qe values.
 template <typename OffT>
 __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
-                         const int32_t* __restrict__ q, const uint32_t* __restrict__ qe,
+                         gptr<const int32_t> q, gptr<const uint32_t> qe,
                          unsigned nf, uint32_t E, uint32_t eb, uint32_t ee) {
   const unsigned lane = lane_id();
   const int k = a.k;
@@ -267,7 +292,7 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
       if (valid) {
         v = a.adj[rsj + (OffT)(my - qoj)];
         if (a.depth[v] > k) {
-          const int old = atomicMax(&a.parent[v], u);
+          const int old = atomicMax(gen(&a.parent[v]), u);
           if (old == -1) {
             fresh = true;
             a.depth[v] = k + 1;
@@ -278,10 +303,10 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
         uint32_t deg = 0;
         if (fresh) {
           deg = (uint32_t)(a.off[v + 1] - a.off[v]);
-          atomicOr(reinterpret_cast<unsigned*>(a.fb_next) + (v >> 5), 1u << (v & 31));
+          atomicOr(reinterpret_cast<unsigned*>(gen(a.fb_next)) + (v >> 5), 1u << (v & 31));
         }
         const unsigned long long z = __ballot(fresh && deg == 0);
-        if (z && lane == 0) atomicAdd(a.d0_next, (unsigned)__popcll(z));
+        if (z && lane == 0) atomicAdd(gen(a.d0_next), (unsigned)__popcll(z));
         stage_push(a, sb, stg, fresh && deg > 0, v, deg);
       }
     }
@@ -329,7 +354,7 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
 // loads; returns the first hit (or -1) and moves p down past them.
 template <int N, typename OffT>
 __device__ __forceinline__ int scan_top(const LevelView<OffT>& a, OffT lo, OffT& p) {
-  const unsigned* __restrict__ fbc = a.fb_cur;
+  const gptr<const unsigned> fbc = a.fb_cur;
   const OffT have = p - lo;
   int c[N];
 #pragma unroll
@@ -352,7 +377,7 @@ __device__ __forceinline__ void scan_rows_together(const LevelView<OffT>& a,
                                                    unsigned long long left, OffT lo, OffT p,
                                                    int& found) {
   const unsigned lane = lane_id();
-  const unsigned* __restrict__ fbc = a.fb_cur;
+  const gptr<const unsigned> fbc = a.fb_cur;
   while (left) {
     const int L = __builtin_ctzll(left);
     left &= left - 1;
@@ -382,7 +407,7 @@ __device__ __forceinline__ void scan_rows_together(const LevelView<OffT>& a,
 template <typename OffT>
 __device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine, OffT lo, OffT hi) {
   const unsigned lane = lane_id();
-  const unsigned* __restrict__ fbc = a.fb_cur;
+  const gptr<const unsigned> fbc = a.fb_cur;
   int found = -1;
   const bool serial = mine && (hi - lo) <= (OffT)kWave;
   if (serial) {
@@ -442,7 +467,7 @@ __device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb
     }
     const int pw = __shfl_up(wi, 1);
     const bool head = lane == 0 || pw != wi;
-    if (head && bits) atomicOr(&a.fb_next[wi], bits);
+    if (head && bits) atomicOr(gen(&a.fb_next[wi]), bits);
   }
   stage_push(a, sb, stg, nw, v, deg);
 }
@@ -518,8 +543,7 @@ template <typename OffT, bool kSorted>
 __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L) {
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
-  const unsigned long long* __restrict__ fbc64 =
-      reinterpret_cast<const unsigned long long*>(a.fb_cur);
+  const gptr<const unsigned long long> fbc64 = (gptr<const unsigned long long>)a.fb_cur;
   const long long W = num_waves();
   const long long gw = global_wave();
   const long long b0 = a.bu_bounds ? (long long)a.bu_bounds[gw] : a.nwords * gw / W;
@@ -633,9 +657,9 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   {
     // F_{k+2}'s bitmap held F_{k-1}.  A top-down level also folds F_k into
     // the visited bitmap, which only bottom-up levels read.
-    unsigned long long* z = A->fb[c2];
-    const unsigned long long* f = A->fb[c0];
-    unsigned long long* vis = A->visited;
+    const gptr<unsigned long long> z = glob(A->fb[c2]);
+    const gptr<const unsigned long long> f = glob((const unsigned long long*)A->fb[c0]);
+    const gptr<unsigned long long> vis = glob(A->visited);
     const bool fold = mode == kTopDown && A->symmetric && A->td_to_bu_div > 0;
     const long long nwords = A->nwords;
     const long long stride = (long long)gridDim.x * blockDim.x;
@@ -649,26 +673,26 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   }
   const int sq = blockIdx.x % kSubQueues;
   LevelView<OffT> a;
-  a.off = A->off;
-  a.adj = A->adj;
-  a.parent = A->parent;
-  a.depth = A->depth;
-  a.visited = A->visited;
-  a.fb_cur = reinterpret_cast<const unsigned*>(A->fb[c0]);
-  a.fb_next = A->fb[c1];
+  a.off = glob(A->off);
+  a.adj = glob(A->adj);
+  a.parent = glob(A->parent);
+  a.depth = glob(A->depth);
+  a.visited = glob(A->visited);
+  a.fb_cur = glob(reinterpret_cast<const unsigned*>(A->fb[c0]));
+  a.fb_next = glob(A->fb[c1]);
   a.qcap = A->qcap;
-  a.q_cur = A->q[k & 1];
-  a.qe_cur = A->qe[k & 1];
-  a.q_next = A->q[(k + 1) & 1] + sq * a.qcap;
-  a.qe_next = A->qe[(k + 1) & 1] + sq * a.qcap;
-  a.ctr_next = &st->ctr[c1][sq];
-  a.d0_next = &st->d0[c1];
-  a.ctr_cur = st->ctr[c0];
+  a.q_cur = glob(A->q[k & 1]);
+  a.qe_cur = glob(A->qe[k & 1]);
+  a.q_next = glob(A->q[(k + 1) & 1] + sq * a.qcap);
+  a.qe_next = glob(A->qe[(k + 1) & 1] + sq * a.qcap);
+  a.ctr_next = glob(&st->ctr[c1][sq]);
+  a.d0_next = glob(&st->d0[c1]);
+  a.ctr_cur = glob(st->ctr[c0]);
   // The first bottom-up level after top-down meets almost every vertex
   // unreached; its per-vertex cost grows with the degree, so the shares are
   // cut at equal sums of 1 + log2(1 + degree).  Later bottom-up levels meet
   // few, low-degree vertices and split the words evenly.
-  a.bu_bounds = (prev == kTopDown) ? A->bu_bounds : nullptr;
+  a.bu_bounds = glob((prev == kTopDown) ? A->bu_bounds : nullptr);
   a.n = A->n;
   a.nwords = A->nwords;
   a.k = k;
@@ -739,92 +763,349 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
 // outputs follow the single-source rule (largest neighbour one level up), so
 // a batch returns exactly what separate BFS calls return.
 //
-//   * push level k: every edge (u,v) of the union frontier ORs
-//     fr_k[u] & ~reached(v) into fr_{k+1}[v]; the lane whose atomicOr saw 0
-//     appends v.  The parents of those bits are found in level k+1's kernel
-//     by scanning v's sorted row from the top against fr_k (symmetric graph:
-//     the row holds v's in-neighbours), clearing each bit at its first hit.
-//   * pull level k: every vertex with sources still unreached scans its row
-//     from the top against fr_k, assigning parents as bits hit, until all of
-//     its unreached bits are found or the row ends.
-// "Reached" bits of v during level k are vis[v] | fr_k[v]: vis gets level-k
-// bits during level k (resolve), so the union keeps the guard exact.
+// "Reached by level k" is vis[v] | F_k[v]: level k's kernel ORs F_k into vis
+// for the queue entries of F_k while other waves read it, and the union is
+// the same before and after that store.
+//
+//   * push level k (small union frontier): every edge (u,v) of the frontier
+//     ORs F_k[u] & ~reached(v) into F_{k+1}[v] (the lane whose atomicOr saw 0
+//     appends v) and does atomicMax(&parent[i][v], u) for each of those
+//     bits i, as the single-source top-down claim does.
+//   * pull level k: the work axis (edges plus a fixed cost per row) is cut
+//     into equal slices, one per wavefront; every vertex with sources not yet
+//     reached scans its row (its part of the row, when a slice boundary cuts
+//     it) against F_k.
+//
+// The pull walks rows in windows of kWinRows rows, 64 x kUnroll edges per
+// pass, each row from its largest neighbour down (rows are sorted), so the
+// first hit of a source bit in a row is its parent.  The first hit is found
+// with a segmented prefix-OR over the lanes holding that row's edges; a row
+// leaves the scan once every bit it needs is found.  Parents of a window go
+// to an LDS tile [row][source] and leave as one coalesced store per source
+// bit.  A row cut by a slice boundary is scanned by several wavefronts, each
+// combining its local first hits with atomicMax (and atomicOr for F_{k+1}).
+// Unsorted rows are scanned completely, every hit taking a max.
+//
+// Mask layout: the F arrays are stored by degree rank (rank[v] = position of
+// v in decreasing-degree order), and adj_rank[e] = rank[adj[e]] is a second
+// copy of the adjacency in rank ids.  The scan gathers F[adj_rank[e]] once per
+// edge, and on skewed graphs most of those gathers hit the few high-degree
+// vertices, whose masks are then packed into a few MB of L2 instead of one
+// cache line each.  vis, the queues and the outputs keep the original ids;
+// a hit reads inv_rank[rank] for the parent's original id (parents are
+// mostly high-degree vertices, so those reads stay in L2; reading adj[e]
+// instead cost one HBM line per hit).
+constexpr int kWinRows = 32;      // rows per window (lanes 0..31 own one each)
+constexpr int kTileStride = 65;   // LDS words per tile row: column reads hit distinct banks
+#ifndef PBBS_BATCH_UNROLL
+#define PBBS_BATCH_UNROLL 4
+#endif
+constexpr int kUnroll = PBBS_BATCH_UNROLL;   // 64-edge groups in flight per pass
+constexpr int kPass = kWave * kUnroll;
+// Work units are cut at equal sums of (degree + kRowCost) per row: a row
+// costs about as much as kRowCost edges (its window slot, the mask and
+// offset loads, the output write-out) whatever its degree, and splitting by
+// edges alone gave the wavefronts over low-degree ranges ~30x the rows.
+constexpr int kRowCost = 16;
+// A pull level tests the frontier bitmap before each mask gather when
+// |F_k| < n / kFilterDiv.
+constexpr int kFilterDiv = 4;
+
 template <typename OffT>
 struct MbfsView {
-  const OffT* __restrict__ off;
-  const int32_t* __restrict__ adj;
-  int32_t* parent;
-  int32_t* depth;
-  unsigned long long* vis;
-  const unsigned long long* fr_prev;   // F_{k-1}
-  const unsigned long long* fr_cur;    // F_k
-  unsigned long long* fr_next;         // F_{k+1}
-  const int32_t* q_cur;
-  const uint32_t* qe_cur;
-  const unsigned long long* ctr_cur;
-  int32_t* q_next;                     // this workgroup's sub-queue region
-  uint32_t* qe_next;
-  unsigned long long* ctr_next;
+  gptr<const OffT> off;
+  gptr<const int32_t> adj;
+  gptr<const int32_t> adj_rank;
+  gptr<const int32_t> rank;
+  gptr<const int32_t> inv_rank;
+  gptr<int32_t> parent;
+  gptr<int32_t> depth;
+  gptr<unsigned long long> vis;
+  gptr<const unsigned long long> fr_cur;    // F_k, by rank
+  gptr<unsigned long long> fr_next;         // F_{k+1}
+  gptr<const unsigned> bm_cur;              // bitmaps (by rank) of the non-zero masks
+  gptr<unsigned> bm_next;
+  gptr<const int32_t> q_cur;
+  gptr<const uint32_t> qe_cur;
+  gptr<const unsigned long long> ctr_cur;
+  gptr<int32_t> q_next;                     // this workgroup's sub-queue region
+  gptr<uint32_t> qe_next;
+  gptr<unsigned long long> ctr_next;
+  gptr<const int32_t> pull_rows;            // first row of each wavefront's edge slice
   long long n;
-  long long nwords;
+  long long m;
   long long qcap;
   unsigned long long srcmask;
   int k;
 };
 
+struct MbfsLds {
+  BatchStageBuf sb;
+  int32_t tile[kWavesPerBlock][kWinRows * kTileStride];
+};
+
+// One row of a window, held by lane p < kWinRows.
 template <typename OffT>
-__device__ __forceinline__ void assign_sources(const MbfsView<OffT>& a, int v, int u,
-                                               unsigned long long bits, int d) {
-  while (bits) {
-    const long long i = __builtin_ctzll(bits);
-    bits &= bits - 1;
-    a.parent[i * a.n + v] = u;
-    a.depth[i * a.n + v] = d;
+struct WinRow {
+  int v;                      // vertex
+  OffT lo, hi;                // adj[lo, hi) still to scan, from hi - 1 down
+  unsigned long long rem;     // sources this row looks for
+  unsigned long long found;   // sources found so far (this level)
+  bool split;                 // a slice boundary cuts the row
+};
+
+__device__ __forceinline__ unsigned long long shfl64(unsigned long long x, int src) {
+  return __shfl(x, src);
+}
+
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long x) {
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) x |= __shfl_xor(x, d);
+  return x;
+}
+
+// Cross-lane moves through DPP (a VALU operand modifier, a few cycles)
+// instead of ds_bpermute (an LDS round trip).  `old` is what lanes without a
+// source lane (or masked rows) receive.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned dpp_u32(unsigned x, unsigned old) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)x, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int dpp_i32(int x, int old) {
+  return __builtin_amdgcn_update_dpp(old, x, kCtrl, kRowMask, 0xf, false);
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// One step of a keyed (segmented) inclusive OR scan: lanes take the value
+// of the source lane when it carries the same key.  Keys are non-decreasing
+// over the lanes, so equal keys at both ends mean one segment in between.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void seg_or_step(unsigned long long& x, int key) {
+  const unsigned lo = dpp_u32<kCtrl, kRowMask>((unsigned)x, 0u);
+  const unsigned hi = dpp_u32<kCtrl, kRowMask>((unsigned)(x >> 32), 0u);
+  const int k2 = dpp_i32<kCtrl, kRowMask>(key, INT32_MIN);
+  if (k2 == key) x |= ((unsigned long long)hi << 32) | lo;
+}
+
+// Inclusive OR of x over the lanes at or below this one with the same key.
+__device__ __forceinline__ unsigned long long seg_or_scan(unsigned long long x, int key) {
+  seg_or_step<kRowShr1, 0xf>(x, key);
+  seg_or_step<kRowShr2, 0xf>(x, key);
+  seg_or_step<kRowShr4, 0xf>(x, key);
+  seg_or_step<kRowShr8, 0xf>(x, key);
+  seg_or_step<kRowBcast15, 0xa>(x, key);
+  seg_or_step<kRowBcast31, 0xc>(x, key);
+  return x;
+}
+
+// The value of the lane below (0 for lane 0 or a different key).
+__device__ __forceinline__ unsigned long long seg_prev(unsigned long long x, int key) {
+  const unsigned lo = dpp_u32<kWaveShr1, 0xf>((unsigned)x, 0u);
+  const unsigned hi = dpp_u32<kWaveShr1, 0xf>((unsigned)(x >> 32), 0u);
+  const int k2 = dpp_i32<kWaveShr1, 0xf>(key, INT32_MIN);
+  return k2 == key ? (((unsigned long long)hi << 32) | lo) : 0ull;
+}
+
+// Inclusive sum over the wavefront through DPP.
+__device__ __forceinline__ unsigned wave_inclusive_sum_dpp(unsigned x) {
+  x += dpp_u32<kRowShr1, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr2, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr4, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr8, 0xf>(x, 0u);
+  x += dpp_u32<kRowBcast15, 0xa>(x, 0u);
+  x += dpp_u32<kRowBcast31, 0xc>(x, 0u);
+  return x;
+}
+
+// One pass of a window scan: which row and adjacency position each lane
+// takes (64 x kUnroll positions, rows in lane order), and the rank ids read
+// there.
+template <typename OffT>
+struct ScanPass {
+  uint32_t excl;   // lane p < kWinRows: first position of row p
+  uint32_t cnt;    // lane p: positions row p takes
+  uint32_t total;  // positions in use (wave-uniform)
+  int rowp[kUnroll];
+  OffT te[kUnroll];
+  int ur[kUnroll];
+};
+
+// Assigns the next positions of the rows that still look for sources and
+// issues their adjacency loads.  Rows are walked down from r.hi.
+template <typename OffT>
+__device__ __forceinline__ void scan_assign(const MbfsView<OffT>& a, const WinRow<OffT>& r,
+                                            unsigned long long need, ScanPass<OffT>& P) {
+  const int lane = (int)lane_id();
+  P.cnt = (need != 0 && r.hi > r.lo) ? (uint32_t)min((OffT)(r.hi - r.lo), (OffT)kPass) : 0u;
+  const uint32_t incl = wave_inclusive_sum_dpp(P.cnt);
+  P.total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+  P.excl = incl - P.cnt;
+#pragma unroll
+  for (int s = 0; s < kUnroll; ++s) {
+    const uint32_t pos = (uint32_t)(s * kWave + lane);
+    int p = 0;
+#pragma unroll
+    for (int st = kWinRows / 2; st >= 1; st >>= 1) {
+      const uint32_t e = __shfl(P.excl, p + st);
+      if (e <= pos) p += st;
+    }
+    const OffT hp = shfl_off(r.hi, p);
+    const uint32_t ep = __shfl(P.excl, p);
+    const bool valid = pos < P.total;
+    P.rowp[s] = valid ? p : -1;
+    P.te[s] = hp - 1 - (OffT)(pos - ep);
+    P.ur[s] = valid ? a.adj_rank[P.te[s]] : -1;
   }
 }
 
-// Parents of the bits F_k[v] for the queue entries of F_k (found by a push).
+// The positions a pass took come off the top of each row.
 template <typename OffT>
-__device__ void mbfs_resolve(const MbfsView<OffT>& a, unsigned long long total) {
+__device__ __forceinline__ void scan_consume(WinRow<OffT>& r, const ScanPass<OffT>& P) {
+  const long long used = min((long long)P.excl + P.cnt, (long long)kPass) - (long long)P.excl;
+  if (used > 0) r.hi -= (OffT)used;
+}
+
+// Scans the window's rows against the masks M; parents go to the tile (or,
+// for split rows, straight to the outputs with atomicMax) with depth d.
+// bm, when given, is the bitmap (by rank) of the vertices with a non-zero
+// mask in M: a neighbour outside it is skipped without gathering its mask.
+//
+// Software-pipelined: pass p+1's positions are assigned, and its adjacency
+// loads issued, before pass p's masks are gathered and its hits processed.
+// Pass p+1 is assigned as if no row finished in pass p; a row that did finish
+// finds nothing more in it (its need is 0) and drops out of pass p+2.
+template <typename OffT, bool kSorted>
+__device__ void scan_window(const MbfsView<OffT>& a, gptr<const unsigned long long> M,
+                            gptr<const unsigned> bm, WinRow<OffT>& r, int32_t* tile, int d) {
+  const int lane = (int)lane_id();
+  if (!kSorted) {
+    // every hit takes a max, so the tile cells this window uses start at -1
+    for (unsigned long long b = r.rem; b; b &= b - 1)
+      tile[lane * kTileStride + __builtin_ctzll(b)] = -1;
+    wave_lds_sync();
+  }
+  ScanPass<OffT> cur, nxt;
+  scan_assign(a, r, r.rem, cur);
+  if (cur.total == 0) return;
+  scan_consume(r, cur);
+  for (;;) {
+    const unsigned long long need = kSorted ? (r.rem & ~r.found) : r.rem;
+    scan_assign(a, r, need, nxt);
+    unsigned long long h[kUnroll];
+    if (bm) {
+#pragma unroll
+      for (int s = 0; s < kUnroll; ++s)
+        if (cur.ur[s] >= 0 && !((bm[cur.ur[s] >> 5] >> (cur.ur[s] & 31)) & 1u)) cur.ur[s] = -1;
+    }
+#pragma unroll
+    for (int s = 0; s < kUnroll; ++s) {
+      const unsigned long long np = shfl64(need, cur.rowp[s] < 0 ? 0 : cur.rowp[s]);
+      h[s] = cur.ur[s] >= 0 ? (M[cur.ur[s]] & np) : 0ull;
+    }
+#pragma unroll
+    for (int s = 0; s < kUnroll; ++s) {
+      if (!__ballot(h[s] != 0ull)) continue;   // no hit in this group: nothing changes
+      const int p = cur.rowp[s];
+      const int ps = p < 0 ? 0 : p;
+      // inclusive OR of h over the lanes of this group holding row p
+      const unsigned long long x = seg_or_scan(h[s], p);
+      const int vp = __shfl(r.v, ps);
+      const int sp = __shfl((int)r.split, ps);
+      unsigned long long nb;
+      if (kSorted)
+        nb = h[s] & ~seg_prev(x, p) & ~shfl64(r.found, ps);
+      else
+        nb = h[s];
+      if (nb) {
+        const int u = a.inv_rank[cur.ur[s]];
+        for (unsigned long long b = nb; b; b &= b - 1) {
+          const int i = __builtin_ctzll(b);
+          if (sp) {
+            const long long o = (long long)i * a.n + vp;
+            atomicMax(gen(&a.parent[o]), u);
+            a.depth[o] = d;
+          } else if (kSorted) {
+            tile[p * kTileStride + i] = u;
+          } else {
+            atomicMax(&tile[p * kTileStride + i], u);
+          }
+        }
+      }
+      // the last lane of row q's segment in this group holds the row's OR
+      const long long g0 = (long long)s * kWave;
+      const long long f = max((long long)cur.excl, g0);
+      const long long e = min((long long)cur.excl + cur.cnt, g0 + kWave);
+      const int src = e > f ? (int)(e - 1 - g0) : lane;
+      const unsigned long long t = shfl64(x, src);
+      if (e > f) r.found |= t;
+    }
+    if (nxt.total == 0) break;
+    scan_consume(r, nxt);
+    cur = nxt;
+  }
+}
+
+// Stores the window's unsplit rows' parents (from the tile) and depths.
+template <typename OffT>
+__device__ __forceinline__ void write_window(const MbfsView<OffT>& a, const WinRow<OffT>& r,
+                                             const int32_t* tile, int d) {
+  const int lane = (int)lane_id();
+  const unsigned long long my = (lane < kWinRows && !r.split) ? r.found : 0ull;
+  unsigned long long all = wave_or64(my);
+  wave_lds_sync();
+  while (all) {
+    const int i = __builtin_ctzll(all);
+    all &= all - 1;
+    if ((my >> i) & 1ull) {
+      const long long o = (long long)i * a.n + r.v;
+      a.parent[o] = tile[lane * kTileStride + i];
+      a.depth[o] = d;
+    }
+  }
+  wave_lds_sync();
+}
+
+// vis |= F_k over F_k's queue entries.
+template <typename OffT>
+__device__ void mbfs_mark(const MbfsView<OffT>& a, unsigned long long total) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < (long long)total;
        t += stride) {
-    // entry t of the concatenated sub-queues
     long long idx = t;
     int sq = 0;
-    for (; sq < kSubQueues; ++sq) {
+    for (; sq < kSubQueues - 1; ++sq) {
       const long long c = (uint32_t)a.ctr_cur[sq];
       if (idx < c) break;
       idx -= c;
     }
     const int v = a.q_cur[sq * a.qcap + idx];
-    unsigned long long rem = a.fr_cur[v];
-    const unsigned long long bits = rem;
-    OffT lo = a.off[v], p = a.off[v + 1];
-    while (rem && p > lo) {
-      const OffT have = p - lo;
-      int c[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
-      unsigned long long mk[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) mk[j] = c[j] >= 0 ? a.fr_prev[c[j]] : 0ull;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const unsigned long long hit = mk[j] & rem;
-        if (hit) {
-          assign_sources(a, v, c[j], hit, a.k);
-          rem &= ~hit;
-        }
-      }
-      p = have > 4 ? p - 4 : lo;
-    }
-    atomicOr(&a.vis[v], bits);
+    a.vis[v] = a.vis[v] | a.fr_cur[a.rank[v]];
   }
 }
 
+// Largest entry i < nf of a sub-queue with qe[i] + cost*i <= eb (64-ary
+// search; the key is strictly increasing when cost > 0, non-decreasing else).
+__device__ __forceinline__ unsigned find_entry(gptr<const uint32_t> qe, unsigned nf,
+                                               unsigned long long eb, unsigned cost) {
+  const unsigned lane = lane_id();
+  unsigned lo = 0, hi = nf;
+  while (hi - lo > (unsigned)kWave) {
+    const unsigned step = (hi - lo + kWave - 1) / kWave;
+    const unsigned idx = lo + lane * step;
+    const bool pr = idx < hi && qe[idx] + (unsigned long long)cost * idx <= eb;
+    const unsigned cnt = (unsigned)__popcll(__ballot(pr));
+    lo += (cnt - 1) * step;
+    hi = min(hi, lo + step);
+  }
+  const unsigned idx = lo + lane;
+  const bool pr = idx < hi && qe[idx] + (unsigned long long)cost * idx <= eb;
+  return lo + (unsigned)__popcll(__ballot(pr)) - 1;
+}
+
 template <typename OffT>
-__device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
+__device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg,
                           unsigned long long E) {
   const unsigned lane = lane_id();
   const unsigned long long W = (unsigned long long)num_waves();
@@ -840,24 +1121,10 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
     const unsigned long long hi_e = eend < P + Es ? eend : P + Es;
     P += Es;
     if (lo_e >= hi_e) continue;
-    const int32_t* __restrict__ q = a.q_cur + sq * a.qcap;
-    const uint32_t* __restrict__ qe = a.qe_cur + sq * a.qcap;
+    const gptr<const int32_t> q = a.q_cur + sq * a.qcap;
+    const gptr<const uint32_t> qe = a.qe_cur + sq * a.qcap;
     const uint32_t eb = (uint32_t)(lo_e - (P - Es)), ee = (uint32_t)(hi_e - (P - Es));
-    unsigned lo = 0, hi = nf;
-    while (hi - lo > (unsigned)kWave) {
-      const unsigned step = (hi - lo + kWave - 1) / kWave;
-      const unsigned idx = lo + lane * step;
-      const bool pr = idx < hi && qe[idx] <= eb;
-      const unsigned cnt = (unsigned)__popcll(__ballot(pr));
-      lo += (cnt - 1) * step;
-      hi = min(hi, lo + step);
-    }
-    unsigned i;
-    {
-      const unsigned idx = lo + lane;
-      const bool pr = idx < hi && qe[idx] <= eb;
-      i = lo + (unsigned)__popcll(__ballot(pr)) - 1;
-    }
+    unsigned i = find_entry(qe, nf, eb, 0u);
     uint32_t e = eb;
     while (e < ee) {
       const unsigned idx = i + lane;
@@ -869,7 +1136,7 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
         qv = q[idx];
         qo = qe[idx];
         rs = a.off[qv];
-        qm = a.fr_cur[qv];
+        qm = a.fr_cur[a.rank[qv]];
       }
       const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : (uint32_t)Es;
       const uint32_t lim = min(wend, ee);
@@ -885,12 +1152,26 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
         const OffT rsj = shfl_off(rs, j);
         const uint32_t qoj = __shfl(qo, j);
         const unsigned long long um = __shfl(qm, j);
+        const int uj = __shfl(qv, j);
         bool fresh = false;
         int v = 0;
         if (valid) {
-          v = a.adj[rsj + (OffT)(my - qoj)];
-          const unsigned long long nb = um & ~(a.vis[v] | a.fr_cur[v]) & a.srcmask;
-          if (nb) fresh = atomicOr(&a.fr_next[v], nb) == 0ull;
+          const OffT e = rsj + (OffT)(my - qoj);
+          v = a.adj[e];
+          const int rv = a.adj_rank[e];
+          const unsigned long long nb = um & ~(a.vis[v] | a.fr_cur[rv]) & a.srcmask;
+          if (nb) {
+            const unsigned long long old = atomicOr(gen(&a.fr_next[rv]), nb);
+            fresh = old == 0ull;
+            if (fresh) atomicOr(gen(&a.bm_next[rv >> 5]), 1u << (rv & 31));
+            // every frontier edge into a new bit competes; the max stays
+            for (unsigned long long b = nb; b; b &= b - 1) {
+              const int i = __builtin_ctzll(b);
+              const long long o = (long long)i * a.n + v;
+              atomicMax(gen(&a.parent[o]), uj);
+              if (!((old >> i) & 1ull)) a.depth[o] = a.k + 1;   // the bit's first setter
+            }
+          }
         }
         if (__ballot(fresh)) {
           uint32_t deg = 0;
@@ -904,63 +1185,74 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg,
   }
 }
 
-template <typename OffT>
-__device__ void mbfs_pull(const MbfsView<OffT>& a, StageBuf& sb, Stage& stg) {
-  const unsigned lane = lane_id();
-  const long long W = num_waves();
-  const long long gw = global_wave();
-  const long long b0 = a.nwords * gw / W, b1 = a.nwords * (gw + 1) / W;
-  for (long long b = b0; b < b1; ++b) {
-    const long long v = b * kWave + lane;
-    unsigned long long rem = 0;
-    if (v < a.n) rem = ~(a.vis[v] | a.fr_cur[v]) & a.srcmask;
-    if (!__ballot(rem != 0)) continue;
-    unsigned long long found = 0;
-    OffT lo = 0, hi = 0;
-    if (rem) {
-      lo = a.off[v];
-      hi = a.off[v + 1];
-      OffT p = hi;
-      while (rem && p > lo) {
-        const OffT have = p - lo;
-        int c[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
-        unsigned long long mk[4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) mk[j] = c[j] >= 0 ? a.fr_cur[c[j]] : 0ull;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const unsigned long long hit = mk[j] & rem;
-          if (hit) {
-            assign_sources(a, (int)v, c[j], hit, a.k + 1);
-            found |= hit;
-            rem &= ~hit;
-          }
-        }
-        p = have > 4 ? p - 4 : lo;
-      }
-      if (found) {
-        a.fr_next[v] = found;
-        atomicOr(&a.vis[v], found);
+// Pull over this wavefront's slice of the work axis, on which row v spans
+// [off[v] + kRowCost*v, off[v+1] + kRowCost*(v+1)): its edges, then
+// kRowCost units for the row itself.
+template <typename OffT, bool kSorted>
+__device__ void mbfs_pull(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg, int32_t* tile,
+                          gptr<const unsigned> filter) {
+  const int lane = (int)lane_id();
+  const unsigned long long W = (unsigned long long)num_waves();
+  const unsigned long long gw = (unsigned long long)global_wave();
+  const unsigned long long X = (unsigned long long)a.m + (unsigned long long)kRowCost * a.n;
+  const long long x0 = (long long)(X * gw / W);
+  const long long x1 = (long long)(X * (gw + 1) / W);
+  if (x0 >= x1) return;
+  for (long long r0 = a.pull_rows[gw];; r0 += kWinRows) {
+    const long long v = r0 + lane;
+    bool mine = lane < kWinRows && v < a.n;
+    long long o0 = 0, o1 = 0;
+    if (mine) {
+      o0 = (long long)a.off[v];
+      o1 = (long long)a.off[v + 1];
+      mine = o0 + kRowCost * v < x1;
+    }
+    if (!__ballot(mine)) break;
+    WinRow<OffT> r;
+    r.v = (int)v;
+    r.lo = r.hi = 0;
+    r.rem = 0;
+    r.found = 0;
+    r.split = false;
+    int rv = 0;
+    if (mine) {
+      const long long lo = max(o0, x0 - kRowCost * v);
+      const long long hi = min(o1, x1 - kRowCost * v);
+      if (hi > lo) {
+        r.lo = (OffT)lo;
+        r.hi = (OffT)hi;
+        r.split = lo > o0 || hi < o1;
+        rv = a.rank[v];
+        r.rem = a.srcmask & ~(a.vis[v] | a.fr_cur[rv]);
       }
     }
-    stage_push(a, sb, stg, found != 0, (int)v, (uint32_t)(hi - lo));
+    scan_window<OffT, kSorted>(a, a.fr_cur, filter, r, tile, a.k + 1);
+    bool take = false;
+    if (r.found) {
+      if (!r.split) {
+        a.fr_next[rv] = r.found;
+        take = true;
+      } else {
+        take = atomicOr(gen(&a.fr_next[rv]), r.found) == 0ull;
+      }
+      if (take) atomicOr(gen(&a.bm_next[rv >> 5]), 1u << (rv & 31));
+    }
+    stage_push(a, sb, stg, take, (int)v, take ? (uint32_t)(o1 - o0) : 0u);
+    write_window(a, r, tile, a.k + 1);
   }
 }
 
 template <typename OffT>
-__device__ __forceinline__ int mbfs_mode(const MbfsArgs<OffT>& A, int prev, unsigned long long cnt,
+__device__ __forceinline__ int mbfs_mode(const MbfsArgs<OffT>& A, unsigned long long cnt,
                                          unsigned long long E) {
-  if (A.td_to_bu_div <= 0) return kTopDown;
-  if (prev == kTopDown)
-    return (cnt + E > (unsigned long long)A.m / (unsigned)A.td_to_bu_div) ? kBottomUp : kTopDown;
-  return ((long long)cnt > A.n / A.bu_to_td_div) ? kBottomUp : kTopDown;
+  if (A.td_to_bu_div <= 0 || A.pull_div <= 0) return kTopDown;
+  return (cnt + E) * (unsigned long long)A.pull_div > (unsigned long long)A.m ? kBottomUp
+                                                                               : kTopDown;
 }
 
-template <typename OffT>
+template <typename OffT, bool kSorted>
 __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>* __restrict__ A) {
-  __shared__ StageBuf sb;
+  __shared__ MbfsLds lds;
   MbfsState* st = A->st;
   const int k = (int)A->wg_level[blockIdx.x];
   const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
@@ -980,8 +1272,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
     }
     return;
   }
-  const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
-  const int mode = mbfs_mode(*A, prev, cnt, E);
+  const int mode = mbfs_mode(*A, cnt, E);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->modes[k & 3] = mode;
     st->max_depth = k;
@@ -992,40 +1283,100 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   }
   {
     // F_{k+2}'s masks held F_{k-2}: level k reads F_{k-1} and F_k only.
-    unsigned long long* z = A->fr[(k + 2) & 3];
+    const gptr<unsigned long long> z = glob(A->fr[(k + 2) & 3]);
+    const gptr<unsigned long long> zb = glob(A->fbm[(k + 2) & 3]);
+    const long long n = A->n, nw = A->nwords;
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < A->n; v += stride)
+    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
       z[v] = 0;
+      if (v < nw) zb[v] = 0;
+    }
   }
   const int sq = blockIdx.x % kSubQueues;
   MbfsView<OffT> a;
-  a.off = A->off;
-  a.adj = A->adj;
-  a.parent = A->parent;
-  a.depth = A->depth;
-  a.vis = A->vis;
-  a.fr_prev = A->fr[(k + 3) & 3];
-  a.fr_cur = A->fr[k & 3];
-  a.fr_next = A->fr[(k + 1) & 3];
+  a.off = glob(A->off);
+  a.adj = glob(A->adj);
+  a.adj_rank = glob(A->adj_rank);
+  a.rank = glob(A->rank);
+  a.inv_rank = glob(A->inv_rank);
+  a.parent = glob(A->parent);
+  a.depth = glob(A->depth);
+  a.vis = glob(A->vis);
+  a.fr_cur = glob(A->fr[k & 3]);
+  a.fr_next = glob(A->fr[(k + 1) & 3]);
+  a.bm_cur = glob((const unsigned*)A->fbm[k & 3]);
+  a.bm_next = glob((unsigned*)A->fbm[(k + 1) & 3]);
   a.qcap = A->qcap;
-  a.q_cur = A->q[k & 1];
-  a.qe_cur = A->qe[k & 1];
-  a.ctr_cur = st->ctr[c0];
-  a.q_next = A->q[(k + 1) & 1] + sq * a.qcap;
-  a.qe_next = A->qe[(k + 1) & 1] + sq * a.qcap;
-  a.ctr_next = &st->ctr[c1][sq];
+  a.q_cur = glob(A->q[k & 1]);
+  a.qe_cur = glob(A->qe[k & 1]);
+  a.ctr_cur = glob(st->ctr[c0]);
+  a.q_next = glob(A->q[(k + 1) & 1] + sq * a.qcap);
+  a.qe_next = glob(A->qe[(k + 1) & 1] + sq * a.qcap);
+  a.ctr_next = glob(&st->ctr[c1][sq]);
+  a.pull_rows = glob(A->pull_rows);
   a.n = A->n;
-  a.nwords = A->nwords;
+  a.m = A->m;
   a.srcmask = A->srcmask;
   a.k = k;
-  if (k > 0 && prev == kTopDown) mbfs_resolve(a, cnt);
+  int32_t* tile = lds.tile[threadIdx.x / kWave];
+  mbfs_mark(a, cnt);
   Stage stg;
   if (mode == kTopDown)
-    mbfs_push(a, sb, stg, E);
+    mbfs_push(a, lds.sb, stg, E);
   else
-    mbfs_pull(a, sb, stg);
-  stage_flush_block(a, sb, stg);
+    // a sparse union frontier is tested by its bitmap before gathering masks
+    mbfs_pull<OffT, kSorted>(a, lds.sb, stg, tile,
+                             cnt * kFilterDiv < (unsigned long long)a.n ? a.bm_cur : nullptr);
+  stage_flush_block(a, lds.sb, stg);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
+}
+
+// Degree ranks: keys[v] = ~deg(v) so an ascending sort is decreasing degree.
+template <typename OffT>
+__global__ void degree_key_kernel(const OffT* __restrict__ off, long long n,
+                                  uint32_t* __restrict__ keys, int32_t* __restrict__ ids) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    keys[v] = ~(uint32_t)(off[v + 1] - off[v]);
+    ids[v] = (int32_t)v;
+  }
+}
+
+__global__ void rank_scatter_kernel(const int32_t* __restrict__ order, long long n,
+                                    int32_t* __restrict__ rank) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride)
+    rank[order[r]] = (int32_t)r;
+}
+
+__global__ void adj_rank_kernel(const int32_t* __restrict__ adj, long long m,
+                                const int32_t* __restrict__ rank, int32_t* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < m; e += stride)
+    out[e] = rank[adj[e]];
+}
+
+// First row of each wavefront's slice of the pull work axis (row v spans
+// [off[v] + kRowCost*v, off[v+1] + kRowCost*(v+1))): out[w] = the row whose
+// span holds floor(X*w/W), X = m + kRowCost*n; out[W] = n.
+template <typename OffT>
+__global__ void pull_rows_kernel(const OffT* __restrict__ off, long long n, long long m,
+                                 long long waves, int32_t* __restrict__ out) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > waves) return;
+  const unsigned long long X = (unsigned long long)m + (unsigned long long)kRowCost * n;
+  const unsigned long long x = X * (unsigned long long)w / (unsigned long long)waves;
+  if (w == waves || x >= X) {
+    out[w] = (int32_t)n;
+    return;
+  }
+  long long lo = 0, hi = n - 1;   // largest r with off[r] + kRowCost*r <= x
+  while (lo < hi) {
+    const long long mid = (lo + hi + 1) / 2;
+    if ((unsigned long long)off[mid] + (unsigned long long)kRowCost * mid <= x) lo = mid;
+    else hi = mid - 1;
+  }
+  out[w] = (int32_t)lo;
 }
 
 // Clears the masks and presets every output to "unreached".
@@ -1039,6 +1390,12 @@ __global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
     a.fr[1][v] = 0;
     a.fr[2][v] = 0;
     a.fr[3][v] = 0;
+  }
+  for (long long w = tid; w < a.nwords; w += nth) {
+    a.fbm[0][w] = 0;
+    a.fbm[1][w] = 0;
+    a.fbm[2][w] = 0;
+    a.fbm[3][w] = 0;
   }
   const long long total = (long long)a.nsrc * a.n;
   for (long long t = tid; t < total; t += nth) {
@@ -1059,8 +1416,10 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
     const unsigned long long bit = 1ull << t;
     a.parent[(long long)t * a.n + s] = s;
     a.depth[(long long)t * a.n + s] = 0;
+    const int rs = a.rank[s];
     atomicOr(&a.vis[s], bit);
-    first[t] = atomicOr(&a.fr[0][s], bit) == 0ull;
+    first[t] = atomicOr(&a.fr[0][rs], bit) == 0ull;
+    atomicOr(reinterpret_cast<unsigned*>(a.fbm[0]) + (rs >> 5), 1u << (rs & 31));
   }
   __syncthreads();
   if (t == 0) {
@@ -1371,13 +1730,72 @@ hipError_t launch_mbfs_setup(const MbfsArgs<OffT>& a, MbfsArgs<OffT>* dst, int g
 }
 
 template <typename OffT>
-const void* mbfs_level_symbol() {
-  return reinterpret_cast<const void*>(&mbfs_level_kernel<OffT>);
+const void* mbfs_level_symbol(bool sorted) {
+  return sorted ? reinterpret_cast<const void*>(&mbfs_level_kernel<OffT, true>)
+                : reinterpret_cast<const void*>(&mbfs_level_kernel<OffT, false>);
 }
 
 template <typename OffT>
-hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(mbfs_level_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, dargs);
+hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
+                             hipStream_t stream) {
+  if (sorted)
+    hipLaunchKernelGGL((mbfs_level_kernel<OffT, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+  else
+    hipLaunchKernelGGL((mbfs_level_kernel<OffT, false>), dim3(grid), dim3(kBlock), 0, stream,
+                       dargs);
+  return hipGetLastError();
+}
+
+// rank[v] = position of v in decreasing-degree order (ties by id),
+// inv_rank[r] = the vertex of rank r, and adj_rank[e] = rank[adj[e]].
+// Synchronises the stream.
+template <typename OffT>
+hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, long long m,
+                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int grid,
+                              hipStream_t stream) {
+  uint32_t *keys = nullptr, *keys2 = nullptr;
+  int32_t *ids = nullptr;
+  int32_t* order = inv_rank;
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+  hipError_t e = hipMalloc(&keys, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&keys2, n * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&ids, n * sizeof(int32_t));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(degree_key_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, n, keys,
+                       ids);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, ids, order, (int)n, 0,
+                                           32, stream);
+  if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
+  if (e == hipSuccess)
+    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, order, (int)n, 0, 32,
+                                           stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(rank_scatter_kernel, dim3(grid), dim3(kBlock), 0, stream, order, n, rank);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess && m > 0) {
+    hipLaunchKernelGGL(adj_rank_kernel, dim3(grid), dim3(kBlock), 0, stream, adj, m, rank,
+                       adj_rank);
+    e = hipGetLastError();
+  }
+  const hipError_t se = hipStreamSynchronize(stream);
+  (void)hipFree(keys);
+  (void)hipFree(keys2);
+  (void)hipFree(ids);
+  (void)hipFree(tmp);
+  return e != hipSuccess ? e : se;
+}
+
+template <typename OffT>
+hipError_t launch_pull_rows(const OffT* off, long long n, long long m, long long waves,
+                            int32_t* out, hipStream_t stream) {
+  const long long threads = waves + 1;
+  hipLaunchKernelGGL(pull_rows_kernel<OffT>, dim3((unsigned)((threads + kBlock - 1) / kBlock)),
+                     dim3(kBlock), 0, stream, off, n, m, waves, out);
   return hipGetLastError();
 }
 
@@ -1411,8 +1829,12 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
                                               unsigned long long*, int, hipStream_t);            \
   template hipError_t launch_mbfs_setup<OffT>(const MbfsArgs<OffT>&, MbfsArgs<OffT>*, int,      \
                                              hipStream_t);                                      \
-  template const void* mbfs_level_symbol<OffT>();                                               \
-  template hipError_t launch_mbfs_level<OffT>(const MbfsArgs<OffT>*, int, hipStream_t);          \
+  template const void* mbfs_level_symbol<OffT>(bool);                                           \
+  template hipError_t launch_mbfs_level<OffT>(const MbfsArgs<OffT>*, bool, int, hipStream_t);    \
+  template hipError_t launch_pull_rows<OffT>(const OffT*, long long, long long, long long,       \
+                                             int32_t*, hipStream_t);                             \
+  template hipError_t build_degree_ranks<OffT>(const OffT*, const int32_t*, long long, long long, \
+                                               int32_t*, int32_t*, int32_t*, int, hipStream_t);  \
   template hipError_t launch_mbfs_stats<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
   template hipError_t launch_word_weights<OffT>(const OffT*, long long, unsigned long long*, int, \
                                                hipStream_t);                                    \

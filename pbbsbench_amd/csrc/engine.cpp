@@ -37,9 +37,17 @@ template <typename OffT>
 hipError_t launch_mbfs_setup(const MbfsArgs<OffT>& a, MbfsArgs<OffT>* dst, int grid,
                              hipStream_t stream);
 template <typename OffT>
-const void* mbfs_level_symbol();
+const void* mbfs_level_symbol(bool sorted);
 template <typename OffT>
-hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, int grid, hipStream_t stream);
+hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
+                             hipStream_t stream);
+template <typename OffT>
+hipError_t launch_pull_rows(const OffT* off, long long n, long long m, long long waves,
+                            int32_t* out, hipStream_t stream);
+template <typename OffT>
+hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, long long m,
+                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int grid,
+                              hipStream_t stream);
 template <typename OffT>
 hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
 template <typename OffT>
@@ -135,8 +143,14 @@ struct pbbs_graph {
   int32_t* d_adj = nullptr;
   unsigned long long* d_iso = nullptr;  // degree-0 bitmap
   unsigned long long* d_wprefix = nullptr;  // exclusive prefix of per-word degree weights
-  uint32_t* d_bounds = nullptr;         // wavefront shares of the weighted bottom-up split
-  int64_t bounds_waves = 0;             // wavefront count d_bounds was cut for
+  // Per launch shape (wavefront count), built on first use and kept until the
+  // graph is freed: contexts with different shapes may share the graph.
+  std::mutex meta_mu;
+  std::map<int64_t, uint32_t*> bounds;   // weighted bottom-up shares (single source)
+  std::map<int64_t, int32_t*> pull_rows; // first row of each edge slice (batched pull)
+  int32_t* d_rank = nullptr;             // batched BFS: decreasing-degree rank of each vertex
+  int32_t* d_inv_rank = nullptr;         // batched BFS: vertex of each rank
+  int32_t* d_adj_rank = nullptr;         // batched BFS: adj in rank ids
   int64_t isolated = 0;
   int64_t max_degree = 0;
   int64_t device_bytes = 0;
@@ -179,6 +193,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->bu_serial_depth = 32;
   t->chunk_levels = 16;
   t->use_graphs = 1;
+  t->batch_pull_div = 10;
   return PBBS_OK;
 }
 
@@ -257,10 +272,10 @@ int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
   if (!c || !t) return fail(PBBS_ERR_INVALID, "NULL argument");
   if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
       t->bu_serial_depth < 0 || t->bu_serial_depth > 4096 || t->chunk_levels < 1 ||
-      t->chunk_levels > 4096)
+      t->chunk_levels > 4096 || t->batch_pull_div < 0)
     return fail(PBBS_ERR_INVALID,
                 "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
-                "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096)");
+                "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096, batch_pull_div>=0)");
   c->tuning = *t;
   return PBBS_OK;
 }
@@ -355,7 +370,11 @@ int pbbs_graph_free(pbbs_graph* g) {
   (void)hipFree(g->d_adj);
   (void)hipFree(g->d_iso);
   (void)hipFree(g->d_wprefix);
-  (void)hipFree(g->d_bounds);
+  for (auto& kv : g->bounds) (void)hipFree(kv.second);
+  for (auto& kv : g->pull_rows) (void)hipFree(kv.second);
+  (void)hipFree(g->d_rank);
+  (void)hipFree(g->d_inv_rank);
+  (void)hipFree(g->d_adj_rank);
   delete g;
   return PBBS_OK;
 }
@@ -466,21 +485,83 @@ static int next_chunk(int base, int launched) {
 }
 
 // Cuts the weighted bottom-up shares for `waves` wavefronts (cached per graph).
-static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves) {
-  if (g->bounds_waves == waves) return PBBS_OK;
-  (void)hipFree(g->d_bounds);
-  g->d_bounds = nullptr;
-  g->bounds_waves = 0;
+static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32_t** out) {
+  std::lock_guard<std::mutex> lk(g->meta_mu);
+  auto it = g->bounds.find(waves);
+  if (it != g->bounds.end()) {
+    *out = it->second;
+    return PBBS_OK;
+  }
   const int64_t nwords = (g->n + kWave - 1) / kWave;
-  HIP_TRY(hipMalloc(&g->d_bounds, (waves + 1) * sizeof(uint32_t)));
-  HIP_TRY(launch_share_bounds(g->d_wprefix, nwords, waves, g->d_bounds, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  g->bounds_waves = waves;
+  uint32_t* d = nullptr;
+  HIP_TRY(hipMalloc(&d, (waves + 1) * sizeof(uint32_t)));
+  hipError_t e = launch_share_bounds(g->d_wprefix, nwords, waves, d, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return fail(PBBS_ERR_HIP, "wavefront shares: %s", hipGetErrorString(e));
+  }
+  g->bounds[waves] = d;
+  *out = d;
   return PBBS_OK;
 }
 
 template <typename OffT>
-static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth) {
+static int ensure_pull_rows(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const int32_t** out) {
+  std::lock_guard<std::mutex> lk(g->meta_mu);
+  auto it = g->pull_rows.find(waves);
+  if (it != g->pull_rows.end()) {
+    *out = it->second;
+    return PBBS_OK;
+  }
+  int32_t* d = nullptr;
+  HIP_TRY(hipMalloc(&d, (waves + 1) * sizeof(int32_t)));
+  hipError_t e = launch_pull_rows<OffT>(static_cast<const OffT*>(g->d_off), g->n, g->m, waves, d,
+                                        c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return fail(PBBS_ERR_HIP, "edge-slice rows: %s", hipGetErrorString(e));
+  }
+  g->pull_rows[waves] = d;
+  *out = d;
+  return PBBS_OK;
+}
+
+// Degree ranks and the rank-id adjacency of a graph, built on its first
+// batched BFS and kept until the graph is freed.
+template <typename OffT>
+static int ensure_ranks(pbbs_ctx* c, pbbs_graph* g) {
+  std::lock_guard<std::mutex> lk(g->meta_mu);
+  if (g->d_rank) return PBBS_OK;
+  int32_t *rank = nullptr, *inv = nullptr, *adj_rank = nullptr;
+  if (hipMalloc(&rank, g->n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&inv, g->n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&adj_rank, std::max<int64_t>(g->m, 1) * sizeof(int32_t)) != hipSuccess) {
+    (void)hipFree(rank);
+    (void)hipFree(inv);
+    return fail(PBBS_ERR_NOMEM, "degree-rank arrays for n=%lld m=%lld failed", (long long)g->n,
+                (long long)g->m);
+  }
+  const hipError_t e =
+      build_degree_ranks<OffT>(static_cast<const OffT*>(g->d_off), g->d_adj, g->n, g->m, rank,
+                               inv, adj_rank, c->num_cus * 8, c->stream);
+  if (e != hipSuccess) {
+    (void)hipFree(rank);
+    (void)hipFree(inv);
+    (void)hipFree(adj_rank);
+    return fail(PBBS_ERR_HIP, "degree ranks: %s", hipGetErrorString(e));
+  }
+  g->d_rank = rank;
+  g->d_inv_rank = inv;
+  g->d_adj_rank = adj_rank;
+  g->device_bytes += (2 * g->n + g->m) * (int64_t)sizeof(int32_t);
+  return PBBS_OK;
+}
+
+template <typename OffT>
+static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth,
+                               const uint32_t* bounds) {
   const int64_t nwords = (g->n + kWave - 1) / kWave;
   BfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
@@ -500,7 +581,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.st = c->w_state;
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
-  a.bu_bounds = g->d_bounds;
+  a.bu_bounds = bounds;
   a.n = g->n;
   a.m = g->m;
   a.nwords = nwords;
@@ -518,7 +599,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
 // drives many contexts at once without blocking.
 
 template <typename OffT>
-static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out);
+static int get_batch_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphExec_t* out);
 
 static int launch_chunk(pbbs_ctx* c) {
   pbbs_ctx::Run& r = c->run;
@@ -529,8 +610,8 @@ static int launch_chunk(pbbs_ctx* c) {
       rc = r.off64 ? get_level_graph<int64_t>(c, r.sorted, r.grid, r.K, &exec)
                    : get_level_graph<uint32_t>(c, r.sorted, r.grid, r.K, &exec);
     else
-      rc = r.off64 ? get_batch_graph<int64_t>(c, r.grid, r.K, &exec)
-                   : get_batch_graph<uint32_t>(c, r.grid, r.K, &exec);
+      rc = r.off64 ? get_batch_graph<int64_t>(c, r.sorted, r.grid, r.K, &exec)
+                   : get_batch_graph<uint32_t>(c, r.sorted, r.grid, r.K, &exec);
     if (rc != PBBS_OK) return rc;
     HIP_TRY(hipGraphLaunch(exec, c->stream));
   } else {
@@ -545,10 +626,10 @@ static int launch_chunk(pbbs_ctx* c) {
       } else {
         if (r.off64)
           HIP_TRY(launch_mbfs_level<int64_t>(static_cast<const MbfsArgs<int64_t>*>(c->b_args),
-                                             r.grid, c->stream));
+                                             r.sorted, r.grid, c->stream));
         else
           HIP_TRY(launch_mbfs_level<uint32_t>(static_cast<const MbfsArgs<uint32_t>*>(c->b_args),
-                                              r.grid, c->stream));
+                                              r.sorted, r.grid, c->stream));
       }
     }
   }
@@ -582,9 +663,10 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
 template <typename OffT>
 static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth) {
   const int grid = level_grid(c, g->off64, g->sorted);
-  int rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
+  const uint32_t* bounds = nullptr;
+  int rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
   if (rc != PBBS_OK) return rc;
-  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth);
+  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth, bounds);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   pbbs_ctx::Run& r = c->run;
@@ -674,9 +756,10 @@ template <typename OffT>
 static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
                        int32_t max_levels, int32_t* count, double* init_ms) {
   const int grid = level_grid(c, g->off64, g->sorted);
-  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock);
+  const uint32_t* bounds = nullptr;
+  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
   if (brc != PBBS_OK) return brc;
-  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth);
+  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth, bounds);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   volatile int32_t* hs = c->host_status;
@@ -773,7 +856,8 @@ static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, b
     (void)hipFree(c->b_masks);
     c->b_masks = nullptr;
     c->cap_bn = 0;
-    if (hipMalloc(&c->b_masks, 5 * n * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMalloc(&c->b_masks, (5 * n + 4 * ((n + kWave - 1) / kWave)) *
+                                   sizeof(unsigned long long)) != hipSuccess)
       return fail(PBBS_ERR_NOMEM, "batched mask buffers for n=%lld failed", (long long)n);
     c->cap_bn = n;
   }
@@ -791,10 +875,10 @@ static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, b
   return PBBS_OK;
 }
 
-static int batch_grid(pbbs_ctx* c, bool off64) {
+static int batch_grid(pbbs_ctx* c, bool off64, bool sorted) {
   if (c->tuning.grid_blocks > 0) return std::min(c->tuning.grid_blocks, kMaxGrid);
   int per_cu = 0;
-  const void* fn = off64 ? mbfs_level_symbol<int64_t>() : mbfs_level_symbol<uint32_t>();
+  const void* fn = off64 ? mbfs_level_symbol<int64_t>(sorted) : mbfs_level_symbol<uint32_t>(sorted);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -802,8 +886,8 @@ static int batch_grid(pbbs_ctx* c, bool off64) {
 }
 
 template <typename OffT>
-static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out) {
-  const auto key = std::make_tuple((int)sizeof(OffT), grid, K);
+static int get_batch_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphExec_t* out) {
+  const auto key = std::make_tuple((int)sizeof(OffT) * 2 + (sorted ? 1 : 0), grid, K);
   auto it = c->batch_graphs.find(key);
   if (it != c->batch_graphs.end()) {
     *out = it->second;
@@ -813,7 +897,8 @@ static int get_batch_graph(pbbs_ctx* c, int grid, int K, hipGraphExec_t* out) {
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
   for (int i = 0; i < K && le == hipSuccess; ++i)
-    le = launch_mbfs_level<OffT>(static_cast<const MbfsArgs<OffT>*>(c->b_args), grid, c->stream);
+    le = launch_mbfs_level<OffT>(static_cast<const MbfsArgs<OffT>*>(c->b_args), sorted, grid,
+                                 c->stream);
   const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
   if (le != hipSuccess || ce != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -841,6 +926,7 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.depth = d_depth;
   a.vis = c->b_masks;
   for (int j = 0; j < 4; ++j) a.fr[j] = c->b_masks + (j + 1) * n;
+  for (int j = 0; j < 4; ++j) a.fbm[j] = c->b_masks + 5 * n + j * ((n + kWave - 1) / kWave);
   a.q[0] = c->w_q;
   a.q[1] = c->w_q + kSubQueues * c->cap_n;
   a.qe[0] = c->w_qe;
@@ -856,12 +942,22 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.nsrc = nsrc;
   a.srcmask = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
   a.td_to_bu_div = c->tuning.td_to_bu_div;
-  a.bu_to_td_div = c->tuning.bu_to_td_div;
+  a.pull_div = c->tuning.batch_pull_div;
   pbbs_ctx::Run& r = c->run;
   r.kind = 1;
   r.off64 = g->off64;
   r.sorted = g->sorted;
-  r.grid = batch_grid(c, g->off64);
+  r.grid = batch_grid(c, g->off64, g->sorted);
+  {
+    const int32_t* rows = nullptr;
+    int rc = ensure_pull_rows<OffT>(c, g, (int64_t)r.grid * kWavesPerBlock, &rows);
+    if (rc == PBBS_OK) rc = ensure_ranks<OffT>(c, g);
+    if (rc != PBBS_OK) return rc;
+    a.pull_rows = rows;
+    a.rank = g->d_rank;
+    a.inv_rank = g->d_inv_rank;
+    a.adj_rank = g->d_adj_rank;
+  }
   r.K = c->tuning.chunk_levels;
   r.launched = 0;
   r.limit = n + 2;
@@ -869,7 +965,7 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
       (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, c->num_cus * 16));
   if (c->tuning.use_graphs) {
     hipGraphExec_t exec = nullptr;
-    const int rc = get_batch_graph<OffT>(c, r.grid, r.K, &exec);
+    const int rc = get_batch_graph<OffT>(c, r.sorted, r.grid, r.K, &exec);
     if (rc != PBBS_OK) return rc;
   }
   volatile int32_t* hs = c->host_status;
@@ -970,7 +1066,7 @@ namespace pbbs {
 int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
   const int64_t nwords = (n + kWave - 1) / kWave;
   int64_t b = 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * n * 8;
-  if (nsrc > 0) b += 5 * n * 8 + 2 * (int64_t)nsrc * n * 4;
+  if (nsrc > 0) b += 5 * n * 8 + 4 * nwords * 8 + 2 * (int64_t)nsrc * n * 4;
   return b;
 }
 

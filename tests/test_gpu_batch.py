@@ -14,9 +14,9 @@ def say(*a):
     print("[case]", *a, flush=True)
 
 
-def check_batch(ctx, g, sources, wide=False, **tune):
+def check_batch(ctx, g, sources, wide=False, assume_symmetric=False, **tune):
     say(f"batch n={g.n} m={g.m} k={len(sources)} wide={wide} tune={tune}")
-    dg = ctx.upload(g, wide_offsets=wide)
+    dg = ctx.upload(g, wide_offsets=wide, assume_symmetric=assume_symmetric)
     try:
         P, D, st = dg.bfs_batch(sources)
     finally:
@@ -62,8 +62,9 @@ def test_batch_torus_and_pairs(ctx):
     check_batch(ctx, g, list(range(12)))
 
 
-@pytest.mark.parametrize("tune", [dict(td_to_bu_div=0), dict(td_to_bu_div=1_000_000,
-                                  bu_to_td_div=1_000_000), dict(use_graphs=0), dict(grid_blocks=3)])
+@pytest.mark.parametrize("tune", [dict(td_to_bu_div=0), dict(batch_pull_div=0),
+                                  dict(batch_pull_div=1_000_000), dict(use_graphs=0),
+                                  dict(grid_blocks=3), dict(grid_blocks=3, batch_pull_div=1)])
 def test_batch_tuning_invariance(tune):
     g = pb.rmat(12, 16 * 2 ** 12, 0.57, 0.19, 0.19, seed=9)
     rng = np.random.default_rng(3)
@@ -86,3 +87,45 @@ def test_batch_rejects_bad_input(ctx):
     with pytest.raises(pb.PbbsError, match="symmetric"):
         dd.bfs_batch([0])
     dd.free()
+
+
+def shuffled_rows(g, seed):
+    """Same graph with every row's entries in random order."""
+    rng = np.random.default_rng(seed)
+    adj = g.adj.copy()
+    off = g.offsets
+    for v in np.flatnonzero(np.diff(off) > 1):
+        seg = adj[off[v]:off[v + 1]]
+        rng.shuffle(seg)
+    return pb.HostGraph(off, adj)
+
+
+@pytest.mark.parametrize("tune", [dict(), dict(batch_pull_div=1_000_000), dict(grid_blocks=2)])
+def test_batch_unsorted_rows(tune):
+    """Rows in arbitrary order: every hit takes a max, no early exit."""
+    g = shuffled_rows(pb.rmat(11, 16 * 2 ** 11, 0.57, 0.19, 0.19, seed=21), 4)
+    rng = np.random.default_rng(2)
+    src = rng.choice(np.flatnonzero(g.degrees() > 0), 33, replace=False)
+    with pb.Context(0) as c:
+        c.set_tuning(**tune)
+        dg = c.upload(g, assume_symmetric=True)
+        assert not dg.info["sorted"]
+        dg.free()
+        check_batch(c, g, src, assume_symmetric=True)
+
+
+@pytest.mark.parametrize("grid", [1, 2, 5])
+def test_batch_hub_rows_split_across_wavefronts(grid):
+    """A star of 3000 leaves plus a path: with few wavefronts the hub's row is
+    cut by slice boundaries in both pull and resolve."""
+    n = 3000 + 50
+    u = np.concatenate([np.zeros(2999, dtype=np.int32), np.arange(3000, 3049, dtype=np.int32),
+                        [2999]])
+    v = np.concatenate([np.arange(1, 3000, dtype=np.int32),
+                        np.arange(3001, 3050, dtype=np.int32), [3000]])
+    g = pb.from_edges(n, u, v)
+    src = [5, 0, 3049, 1234, 2999, 3025, 7, 7]
+    for pull in (1_000_000, 0, 10):
+        with pb.Context(0) as c:
+            c.set_tuning(grid_blocks=grid, batch_pull_div=pull)
+            check_batch(c, g, src)
