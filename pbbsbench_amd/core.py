@@ -344,11 +344,21 @@ class Server:
         self._graph_n[gid.value] = g.n
         return gid.value
 
-    def submit(self, graph_id, source, depth=True):
-        """Queues BFS(source) on a graph; returns the ticket."""
+    def submit(self, graph_id, source, depth=True, out=None):
+        """Queues BFS(source) on a graph; returns the ticket.  `out` may give
+        (parent, depth-or-None) int32 arrays of the graph's n entries to fill,
+        e.g. reused buffers; otherwise fresh arrays are allocated."""
         n = max(self._graph_n.get(graph_id, 1), 1)   # an unknown id is rejected below
-        parent = np.empty(n, dtype=np.int32)
-        dep = np.empty(n, dtype=np.int32) if depth else None
+        if out is not None:
+            parent, dep = out
+            for a in (parent, dep):
+                if a is not None and (a.dtype != np.int32 or a.size != n
+                                      or not a.flags.c_contiguous):
+                    raise ValueError(f"output buffers must be contiguous int32[{n}]")
+            depth = dep is not None
+        else:
+            parent = np.empty(n, dtype=np.int32)
+            dep = np.empty(n, dtype=np.int32) if depth else None
         t = C.c_uint64()
         with self._lock:
             check(lib().pbbs_server_submit(self._h, graph_id, int(source), _ptr(parent),

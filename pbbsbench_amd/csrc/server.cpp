@@ -193,20 +193,40 @@ int start_copy(Slot& sl) {
   rc = pbbs::async_outputs(sl.ctx, &dp, &dd);
   if (rc != PBBS_OK) return rc;
   const int64_t bytes = (int64_t)sl.reqs.size() * sl.n * sizeof(int32_t);
+  bool want_parent = false, want_depth = false;
+  for (const Request& r : sl.reqs) {
+    want_parent |= r.parent != nullptr;
+    want_depth |= r.depth != nullptr;
+  }
   hipStream_t st_ = pbbs::ctx_stream(sl.ctx);
-  if (hipMemcpyAsync(sl.pin_parent, dp, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess ||
-      hipMemcpyAsync(sl.pin_depth, dd, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess ||
+  if ((want_parent &&
+       hipMemcpyAsync(sl.pin_parent, dp, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess) ||
+      (want_depth &&
+       hipMemcpyAsync(sl.pin_depth, dd, bytes, hipMemcpyDeviceToHost, st_) != hipSuccess) ||
       hipEventRecord(sl.copied, st_) != hipSuccess)
     return fail(PBBS_ERR_HIP, "output copy failed");
   sl.phase = 2;
   return PBBS_OK;
 }
 
+// Copies the staged rows into the callers' buffers: pieces of 1M entries,
+// spread over host threads (a batch of large graphs is GBs, and first touches
+// of fresh caller pages fault one by one).
 void deliver(Slot& sl) {
-  for (size_t i = 0; i < sl.reqs.size(); ++i) {
-    const Request& r = sl.reqs[i];
-    if (r.parent) std::memcpy(r.parent, sl.pin_parent + i * sl.n, sl.n * sizeof(int32_t));
-    if (r.depth) std::memcpy(r.depth, sl.pin_depth + i * sl.n, sl.n * sizeof(int32_t));
+  constexpr int64_t kPiece = 1 << 20;
+  const int64_t pieces = (sl.n + kPiece - 1) / kPiece;
+  const int64_t tasks = (int64_t)sl.reqs.size() * pieces * 2;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(8) if (tasks > 4)
+  for (int64_t t = 0; t < tasks; ++t) {
+    const int64_t q = t / (pieces * 2), rest = t % (pieces * 2);
+    const bool dep = rest >= pieces;
+    const int64_t lo = (dep ? rest - pieces : rest) * kPiece;
+    const int64_t len = std::min(kPiece, sl.n - lo);
+    const Request& r = sl.reqs[q];
+    int32_t* dst = dep ? r.depth : r.parent;
+    if (!dst) continue;
+    const int32_t* src = (dep ? sl.pin_depth : sl.pin_parent) + q * sl.n + lo;
+    std::memcpy(dst + lo, src, len * sizeof(int32_t));
   }
 }
 
