@@ -82,6 +82,7 @@ constexpr int kMaxSources = 64;
 
 struct MbfsState {
   unsigned long long ctr[3][kSubQueues];  // union-frontier sub-queue counters, [k % 3]
+  unsigned long long unfinished[3];       // rows a pull left with sources to find, [(k+1) % 3]
   int32_t modes[4];
   int32_t finished;
   int32_t push_levels;
@@ -100,6 +101,7 @@ struct MbfsArgs {
   const int32_t* inv_rank;     // [n] vertex of each rank
   int32_t* parent;             // out: [nsrc][n], source-major
   int32_t* depth;              // out: [nsrc][n]
+  uint8_t* dlev;               // [n][64] level at which each source reached v
   unsigned long long* vis;     // [n] sources that reached v (by vertex id)
   unsigned long long* fr[4];   // [n] each, indexed by rank
   unsigned long long* fbm[4];  // [nwords] each: bit r set iff fr[.][r] != 0
@@ -109,7 +111,9 @@ struct MbfsArgs {
   uint32_t* wg_level;
   int32_t* host_status;
   const int32_t* sources;      // [nsrc]
-  const int32_t* pull_rows;    // first row of each wavefront's edge slice (waves + 1 entries)
+  const int32_t* pull_rows;    // first list entry of each wavefront's pull slice (waves + 1)
+  const int32_t* rows;         // vertices of degree > 0, ascending
+  int64_t nrows;
   int64_t n;
   int64_t m;
   int64_t nwords;

@@ -25,6 +25,7 @@
 // workgroup, spread over kSubQueues counters.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <climits>
 
 #include <hipcub/hipcub.hpp>
 
@@ -91,11 +92,13 @@ __device__ __forceinline__ int wave_max(int x) {
   return x;
 }
 
-// Orders this wavefront's LDS writes before its later LDS reads by other lanes.
+// Orders this wavefront's LDS writes before its later LDS reads by other
+// lanes.  The fences name LDS only: a fence on all memory would also wait for
+// every global store in flight (loads and stores share one counter here).
 __device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 __device__ __forceinline__ uint32_t shfl_off(uint32_t x, int src) { return __shfl(x, src); }
@@ -810,6 +813,11 @@ constexpr int kRowCost = 16;
 // A pull level tests the frontier bitmap before each mask gather when
 // |F_k| < n / kFilterDiv.
 constexpr int kFilterDiv = 4;
+// When the previous pull left fewer than nrows / kFewRowsDiv rows with
+// sources to find, a pull level splits whole rows evenly over the wavefronts:
+// most rows are then skipped, at a cost per row, and an edge-weighted split
+// piles the many rows of low-degree id ranges on a few wavefronts.
+constexpr int kFewRowsDiv = 4;
 
 template <typename OffT>
 struct MbfsView {
@@ -820,6 +828,7 @@ struct MbfsView {
   gptr<const int32_t> inv_rank;
   gptr<int32_t> parent;
   gptr<int32_t> depth;
+  gptr<uint8_t> dlev;                       // [n][64] level found, per vertex and source
   gptr<unsigned long long> vis;
   gptr<const unsigned long long> fr_cur;    // F_k, by rank
   gptr<unsigned long long> fr_next;         // F_{k+1}
@@ -831,7 +840,9 @@ struct MbfsView {
   gptr<int32_t> q_next;                     // this workgroup's sub-queue region
   gptr<uint32_t> qe_next;
   gptr<unsigned long long> ctr_next;
-  gptr<const int32_t> pull_rows;            // first row of each wavefront's edge slice
+  gptr<const int32_t> pull_rows;            // first list entry of each wavefront's slice
+  gptr<const int32_t> rows;                 // vertices of degree > 0, ascending
+  long long nrows;
   long long n;
   long long m;
   long long qcap;
@@ -841,6 +852,7 @@ struct MbfsView {
 
 struct MbfsLds {
   BatchStageBuf sb;
+  unsigned unf[kWavesPerBlock];   // unfinished rows per wavefront
   int32_t tile[kWavesPerBlock][kWinRows * kTileStride];
 };
 
@@ -856,6 +868,31 @@ struct WinRow {
 
 __device__ __forceinline__ unsigned long long shfl64(unsigned long long x, int src) {
   return __shfl(x, src);
+}
+
+// Depths of the batched BFS go to a per-vertex byte table dlev[v][source]
+// during the levels (a row's 64 bytes are contiguous, so a level touches one
+// 64-byte piece per vertex it reaches), and reach depth[source][v] in one
+// coalesced pass at the end (mbfs_depth_kernel).  Levels from kDepthEscape on
+// are written straight to depth, with kDepthEscape in the table.
+constexpr int kDepthEscape = 255;
+
+template <typename OffT>
+__device__ __forceinline__ void put_depth(const MbfsView<OffT>& a, int i, long long v, int d) {
+  if (d < kDepthEscape) {
+    a.dlev[v * kMaxSources + i] = (uint8_t)d;
+  } else {
+    a.dlev[v * kMaxSources + i] = (uint8_t)kDepthEscape;
+    a.depth[(long long)i * a.n + v] = d;
+  }
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// 0xFF in each byte of a 32-bit word whose bit is set in the low 4 bits of b.
+__device__ __forceinline__ unsigned byte_mask4(unsigned b) {
+  return ((b & 1u) ? 0x000000FFu : 0u) | ((b & 2u) ? 0x0000FF00u : 0u) |
+         ((b & 4u) ? 0x00FF0000u : 0u) | ((b & 8u) ? 0xFF000000u : 0u);
 }
 
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long x) {
@@ -956,7 +993,8 @@ __device__ __forceinline__ void scan_assign(const MbfsView<OffT>& a, const WinRo
     const bool valid = pos < P.total;
     P.rowp[s] = valid ? p : -1;
     P.te[s] = hp - 1 - (OffT)(pos - ep);
-    P.ur[s] = valid ? a.adj_rank[P.te[s]] : -1;
+    // streamed once per level: non-temporal, so it does not evict the hot masks
+    P.ur[s] = valid ? __builtin_nontemporal_load(&a.adj_rank[P.te[s]]) : -1;
   }
 }
 
@@ -1025,7 +1063,7 @@ __device__ void scan_window(const MbfsView<OffT>& a, gptr<const unsigned long lo
           if (sp) {
             const long long o = (long long)i * a.n + vp;
             atomicMax(gen(&a.parent[o]), u);
-            a.depth[o] = d;
+            put_depth(a, i, vp, d);
           } else if (kSorted) {
             tile[p * kTileStride + i] = u;
           } else {
@@ -1058,13 +1096,35 @@ __device__ __forceinline__ void write_window(const MbfsView<OffT>& a, const WinR
   while (all) {
     const int i = __builtin_ctzll(all);
     all &= all - 1;
-    if ((my >> i) & 1ull) {
-      const long long o = (long long)i * a.n + r.v;
-      a.parent[o] = tile[lane * kTileStride + i];
-      a.depth[o] = d;
-    }
+    if ((my >> i) & 1ull)
+      __builtin_nontemporal_store(tile[lane * kTileStride + i], &a.parent[(long long)i * a.n + r.v]);
   }
   wave_lds_sync();
+  if (my) {
+    // this row's depth bytes: read, set the bytes of the sources found, write
+    const gptr<u32x4> row = (gptr<u32x4>)(a.dlev + (long long)r.v * kMaxSources);
+    const unsigned dv = d < kDepthEscape ? (unsigned)d : (unsigned)kDepthEscape;
+    const unsigned d4 = dv * 0x01010101u;
+    u32x4 w[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) w[j] = row[j];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const unsigned m0 = byte_mask4((unsigned)(my >> (16 * j)) & 15u);
+      const unsigned m1 = byte_mask4((unsigned)(my >> (16 * j + 4)) & 15u);
+      const unsigned m2 = byte_mask4((unsigned)(my >> (16 * j + 8)) & 15u);
+      const unsigned m3 = byte_mask4((unsigned)(my >> (16 * j + 12)) & 15u);
+      w[j].x = (w[j].x & ~m0) | (d4 & m0);
+      w[j].y = (w[j].y & ~m1) | (d4 & m1);
+      w[j].z = (w[j].z & ~m2) | (d4 & m2);
+      w[j].w = (w[j].w & ~m3) | (d4 & m3);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) row[j] = w[j];
+    if (d >= kDepthEscape)
+      for (unsigned long long b = my; b; b &= b - 1)
+        a.depth[(long long)__builtin_ctzll(b) * a.n + r.v] = d;
+  }
 }
 
 // vis |= F_k over F_k's queue entries.
@@ -1169,7 +1229,7 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg
               const int i = __builtin_ctzll(b);
               const long long o = (long long)i * a.n + v;
               atomicMax(gen(&a.parent[o]), uj);
-              if (!((old >> i) & 1ull)) a.depth[o] = a.k + 1;   // the bit's first setter
+              if (!((old >> i) & 1ull)) put_depth(a, i, v, a.k + 1);   // the bit's first setter
             }
           }
         }
@@ -1185,48 +1245,96 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg
   }
 }
 
-// Pull over this wavefront's slice of the work axis, on which row v spans
-// [off[v] + kRowCost*v, off[v+1] + kRowCost*(v+1)): its edges, then
-// kRowCost units for the row itself.
+// Pull over this wavefront's slice of the work axis.  The rows are the
+// vertices of degree > 0 in ascending order (a degree-0 vertex is never
+// reached from another), and entry i (vertex v) spans
+// [off[v] + kRowCost*i, off[v'] + kRowCost*(i+1)) with v' the next entry:
+// its edges, then kRowCost units for the row itself.
+//
+// by_rows: wavefront w takes whole rows [nrows*w/W, nrows*(w+1)/W) instead.
+// Rows left with sources to find are counted into *unfinished.
 template <typename OffT, bool kSorted>
 __device__ void mbfs_pull(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg, int32_t* tile,
-                          gptr<const unsigned> filter) {
+                          gptr<const unsigned> filter, bool by_rows,
+                          unsigned long long* unfinished, unsigned* unf_lds) {
   const int lane = (int)lane_id();
+  const int w = threadIdx.x / kWave;
   const unsigned long long W = (unsigned long long)num_waves();
   const unsigned long long gw = (unsigned long long)global_wave();
-  const unsigned long long X = (unsigned long long)a.m + (unsigned long long)kRowCost * a.n;
-  const long long x0 = (long long)(X * gw / W);
-  const long long x1 = (long long)(X * (gw + 1) / W);
-  if (x0 >= x1) return;
-  for (long long r0 = a.pull_rows[gw];; r0 += kWinRows) {
-    const long long v = r0 + lane;
-    bool mine = lane < kWinRows && v < a.n;
-    long long o0 = 0, o1 = 0;
-    if (mine) {
-      o0 = (long long)a.off[v];
-      o1 = (long long)a.off[v + 1];
-      mine = o0 + kRowCost * v < x1;
+  const unsigned long long X = (unsigned long long)a.m + (unsigned long long)kRowCost * a.nrows;
+  long long x0, x1, iend;
+  if (by_rows) {
+    x0 = -1;   // rows are whole: no edge slice cuts them
+    x1 = LLONG_MAX;
+    iend = (long long)((unsigned long long)a.nrows * (gw + 1) / W);
+  } else {
+    x0 = (long long)(X * gw / W);
+    x1 = (long long)(X * (gw + 1) / W);
+    iend = a.nrows;
+  }
+  unsigned left = 0;   // rows still looking for sources (lane 0's count)
+  if (lane == 0) unf_lds[w] = 0;
+  // Row data of the next window is loaded while the current one is scanned:
+  // vertex, offsets, reached mask and rank first, the frontier mask (which
+  // needs the rank) after the scan.
+  long long p0 = 0, p1 = 0;
+  unsigned long long pvis = 0, pfk = 0;
+  int prk = 0, pv = 0;
+  auto load_rows = [&](long long ibase) {
+    const long long i = ibase + lane;
+    if (lane < kWinRows && i < a.nrows) {
+      pv = a.rows[i];
+      p0 = (long long)a.off[pv];
+      p1 = (long long)a.off[pv + 1];
+      pvis = a.vis[pv];
+      prk = a.rank[pv];
+    } else {
+      pv = 0;
+      p0 = p1 = 0;
+      pvis = ~0ull;
+      prk = 0;
     }
+  };
+  auto load_mask = [&]() {
+    pfk = 0;
+    if (p1 > p0 && (a.srcmask & ~pvis)) pfk = a.fr_cur[prk];
+  };
+  long long i0 = by_rows ? (long long)((unsigned long long)a.nrows * gw / W)
+                         : (x0 < x1 ? (long long)a.pull_rows[gw] : iend);
+  load_rows(i0);
+  load_mask();
+  for (;; i0 += kWinRows) {
+    const long long i = i0 + lane;
+    const int v = pv;
+    const long long o0 = p0, o1 = p1;
+    const unsigned long long vis = pvis, fk = pfk;
+    const int rv = prk;
+    const bool mine = lane < kWinRows && i < iend && o0 + kRowCost * i < x1;
     if (!__ballot(mine)) break;
+    load_rows(i0 + kWinRows);
     WinRow<OffT> r;
-    r.v = (int)v;
+    r.v = v;
     r.lo = r.hi = 0;
     r.rem = 0;
     r.found = 0;
     r.split = false;
-    int rv = 0;
     if (mine) {
-      const long long lo = max(o0, x0 - kRowCost * v);
-      const long long hi = min(o1, x1 - kRowCost * v);
+      const long long lo = max(o0, x0 - kRowCost * i);
+      const long long hi = min(o1, x1 - kRowCost * i);
       if (hi > lo) {
         r.lo = (OffT)lo;
         r.hi = (OffT)hi;
         r.split = lo > o0 || hi < o1;
-        rv = a.rank[v];
-        r.rem = a.srcmask & ~(a.vis[v] | a.fr_cur[rv]);
+        r.rem = a.srcmask & ~(vis | fk);
       }
     }
+    if (!__ballot(r.rem != 0ull)) {   // every row of the window is done
+      load_mask();
+      continue;
+    }
     scan_window<OffT, kSorted>(a, a.fr_cur, filter, r, tile, a.k + 1);
+    left += (unsigned)__popcll(__ballot((r.rem & ~r.found) != 0ull && r.lo == (OffT)o0));
+    load_mask();
     bool take = false;
     if (r.found) {
       if (!r.split) {
@@ -1237,8 +1345,15 @@ __device__ void mbfs_pull(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg
       }
       if (take) atomicOr(gen(&a.bm_next[rv >> 5]), 1u << (rv & 31));
     }
-    stage_push(a, sb, stg, take, (int)v, take ? (uint32_t)(o1 - o0) : 0u);
+    stage_push(a, sb, stg, take, v, take ? (uint32_t)(o1 - o0) : 0u);
     write_window(a, r, tile, a.k + 1);
+  }
+  if (lane == 0) unf_lds[w] = left;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int j = 0; j < kWavesPerBlock; ++j) t += unf_lds[j];
+    if (t) atomicAdd(unfinished, t);
   }
 }
 
@@ -1279,6 +1394,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
     if (mode == kTopDown) st->push_levels += 1; else st->pull_levels += 1;
 #pragma unroll
     for (int s = 0; s < kSubQueues; ++s) st->ctr[c2][s] = 0;
+    st->unfinished[c2] = 0;
     __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   {
@@ -1301,6 +1417,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   a.inv_rank = glob(A->inv_rank);
   a.parent = glob(A->parent);
   a.depth = glob(A->depth);
+  a.dlev = glob(A->dlev);
   a.vis = glob(A->vis);
   a.fr_cur = glob(A->fr[k & 3]);
   a.fr_next = glob(A->fr[(k + 1) & 3]);
@@ -1314,6 +1431,8 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   a.qe_next = glob(A->qe[(k + 1) & 1] + sq * a.qcap);
   a.ctr_next = glob(&st->ctr[c1][sq]);
   a.pull_rows = glob(A->pull_rows);
+  a.rows = glob(A->rows);
+  a.nrows = A->nrows;
   a.n = A->n;
   a.m = A->m;
   a.srcmask = A->srcmask;
@@ -1324,9 +1443,16 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   if (mode == kTopDown)
     mbfs_push(a, lds.sb, stg, E);
   else
-    // a sparse union frontier is tested by its bitmap before gathering masks
+  {
+    // a sparse union frontier is tested by its bitmap before gathering masks;
+    // after a pull that left few rows unfinished, rows are split evenly
+    const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
+    const bool by_rows =
+        prev == kBottomUp && st->unfinished[c0] * kFewRowsDiv < (unsigned long long)a.nrows;
     mbfs_pull<OffT, kSorted>(a, lds.sb, stg, tile,
-                             cnt * kFilterDiv < (unsigned long long)a.n ? a.bm_cur : nullptr);
+                             cnt * kFilterDiv < (unsigned long long)a.n ? a.bm_cur : nullptr,
+                             by_rows, &st->unfinished[c1], lds.unf);
+  }
   stage_flush_block(a, lds.sb, stg);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
 }
@@ -1356,27 +1482,35 @@ __global__ void adj_rank_kernel(const int32_t* __restrict__ adj, long long m,
     out[e] = rank[adj[e]];
 }
 
-// First row of each wavefront's slice of the pull work axis (row v spans
-// [off[v] + kRowCost*v, off[v+1] + kRowCost*(v+1))): out[w] = the row whose
-// span holds floor(X*w/W), X = m + kRowCost*n; out[W] = n.
+// First list entry of each wavefront's slice of the pull work axis (entry i,
+// vertex rows[i], starts at off[rows[i]] + kRowCost*i): out[w] = the entry
+// whose span holds floor(X*w/W), X = m + kRowCost*nrows; out[W] = nrows.
 template <typename OffT>
-__global__ void pull_rows_kernel(const OffT* __restrict__ off, long long n, long long m,
-                                 long long waves, int32_t* __restrict__ out) {
+__global__ void pull_rows_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ rows,
+                                 long long nrows, long long m, long long waves,
+                                 int32_t* __restrict__ out) {
   const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (w > waves) return;
-  const unsigned long long X = (unsigned long long)m + (unsigned long long)kRowCost * n;
+  const unsigned long long X = (unsigned long long)m + (unsigned long long)kRowCost * nrows;
   const unsigned long long x = X * (unsigned long long)w / (unsigned long long)waves;
-  if (w == waves || x >= X) {
-    out[w] = (int32_t)n;
+  if (w == waves || x >= X || nrows == 0) {
+    out[w] = (int32_t)nrows;
     return;
   }
-  long long lo = 0, hi = n - 1;   // largest r with off[r] + kRowCost*r <= x
+  long long lo = 0, hi = nrows - 1;   // largest i with off[rows[i]] + kRowCost*i <= x
   while (lo < hi) {
     const long long mid = (lo + hi + 1) / 2;
-    if ((unsigned long long)off[mid] + (unsigned long long)kRowCost * mid <= x) lo = mid;
+    if ((unsigned long long)off[rows[mid]] + (unsigned long long)kRowCost * mid <= x) lo = mid;
     else hi = mid - 1;
   }
   out[w] = (int32_t)lo;
+}
+
+__global__ void nonzero_flag_kernel(const unsigned* __restrict__ keys, long long n,
+                                    uint8_t* __restrict__ flags) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride)
+    flags[v] = keys[v] != ~0u;   // key = ~degree
 }
 
 // Clears the masks and presets every output to "unreached".
@@ -1398,10 +1532,8 @@ __global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
     a.fbm[3][w] = 0;
   }
   const long long total = (long long)a.nsrc * a.n;
-  for (long long t = tid; t < total; t += nth) {
-    a.parent[t] = -1;
-    a.depth[t] = (int32_t)a.n;
-  }
+  for (long long t = tid; t < total; t += nth)   // depth: mbfs_depth_kernel
+    __builtin_nontemporal_store(-1, &a.parent[t]);
   for (long long i = tid; i < kMaxGrid; i += nth) a.wg_level[i] = 0;
 }
 
@@ -1415,7 +1547,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
     const int s = a.sources[t];
     const unsigned long long bit = 1ull << t;
     a.parent[(long long)t * a.n + s] = s;
-    a.depth[(long long)t * a.n + s] = 0;
+    a.dlev[(long long)s * kMaxSources + t] = 0;
     const int rs = a.rank[s];
     atomicOr(&a.vis[s], bit);
     first[t] = atomicOr(&a.fr[0][rs], bit) == 0ull;
@@ -1435,8 +1567,10 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
       E += (unsigned long long)(a.off[s + 1] - a.off[s]);
       ++cnt;
     }
-    for (int j = 0; j < 3; ++j)
+    for (int j = 0; j < 3; ++j) {
       for (int q = 0; q < kSubQueues; ++q) st->ctr[j][q] = 0;
+      st->unfinished[j] = 0;
+    }
     st->ctr[0][0] = (E << 32) | cnt;
     for (int j = 0; j < 4; ++j) st->modes[j] = kTopDown;
     st->finished = 0;
@@ -1448,6 +1582,68 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
     __threadfence_system();
     __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// End of a batched BFS: depth[i][v] for every source i and vertex v, from
+// the level table (n where source i never reached v; entries from
+// kDepthEscape on were written during the levels).  A lane takes 4
+// consecutive vertices, so each source's store is 16 bytes per lane and 1 KB
+// per wavefront.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
+  const gptr<const unsigned long long> vis = glob((const unsigned long long*)a.vis);
+  const gptr<const u32x4> dl = glob((const u32x4*)a.dlev);
+  const gptr<int32_t> depth = glob(a.depth);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int ns = a.nsrc;
+  const int32_t nval = (int32_t)a.n;
+  const long long quads = (a.n + 3) / 4;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
+    const long long v0 = q * 4;
+    const bool whole = v0 + 4 <= a.n && (a.n & 3) == 0;   // 16-byte aligned rows of depth
+    for (int c = 0; c < 4; ++c) {   // 16 sources of the 4 vertices at a time
+      unsigned long long rvm[4];
+      unsigned wv[4][4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const long long v = v0 + t;
+        rvm[t] = v < a.n ? (vis[v] & a.srcmask) : 0ull;
+        u32x4 x = {0u, 0u, 0u, 0u};
+        if (rvm[t]) x = dl[v * 4 + c];
+        wv[t][0] = x.x;
+        wv[t][1] = x.y;
+        wv[t][2] = x.z;
+        wv[t][3] = x.w;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int i = 16 * c + 4 * j + b;
+          if (i >= ns) break;
+          int32_t d[4];
+          bool keep[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const unsigned lv = (wv[t][j] >> (8 * b)) & 0xFFu;
+            const bool hit = (rvm[t] >> i) & 1ull;
+            d[t] = hit ? (int32_t)lv : nval;
+            keep[t] = hit && lv == (unsigned)kDepthEscape;   // written during the levels
+          }
+          const long long o = (long long)i * a.n + v0;
+          if (whole && !(keep[0] | keep[1] | keep[2] | keep[3])) {
+            typedef int i32x4 __attribute__((ext_vector_type(4)));
+            const i32x4 dv = {d[0], d[1], d[2], d[3]};
+            __builtin_nontemporal_store(dv, (gptr<i32x4>)(depth + o));
+          } else {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+              if (v0 + t < a.n && !keep[t]) depth[o + t] = d[t];
+          }
+        }
+      }
+    }
   }
 }
 
@@ -1747,12 +1943,13 @@ hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
 }
 
 // rank[v] = position of v in decreasing-degree order (ties by id),
-// inv_rank[r] = the vertex of rank r, and adj_rank[e] = rank[adj[e]].
+// inv_rank[r] = the vertex of rank r, adj_rank[e] = rank[adj[e]], and
+// rows[0..*nrows) = the vertices of degree > 0 in ascending order.
 // Synchronises the stream.
 template <typename OffT>
 hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, long long m,
-                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int grid,
-                              hipStream_t stream) {
+                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int32_t* rows,
+                              long long* nrows, int grid, hipStream_t stream) {
   uint32_t *keys = nullptr, *keys2 = nullptr;
   int32_t *ids = nullptr;
   int32_t* order = inv_rank;
@@ -1782,7 +1979,27 @@ hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, 
                        adj_rank);
     e = hipGetLastError();
   }
+  // the degree-0 test reuses keys (~degree); keys2 is free again for flags
+  uint8_t* flags = reinterpret_cast<uint8_t*>(keys2);
+  int32_t* count = reinterpret_cast<int32_t*>(ids);   // ids is free after the sort
+  void* tmp2 = nullptr;
+  size_t tmp2_bytes = 0;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(nonzero_flag_kernel, dim3(grid), dim3(kBlock), 0, stream, keys, n, flags);
+    e = hipGetLastError();
+  }
+  hipcub::CountingInputIterator<int32_t> it(0);
+  if (e == hipSuccess)
+    e = hipcub::DeviceSelect::Flagged(nullptr, tmp2_bytes, it, flags, rows, count, (int)n, stream);
+  if (e == hipSuccess) e = hipMalloc(&tmp2, tmp2_bytes);
+  if (e == hipSuccess)
+    e = hipcub::DeviceSelect::Flagged(tmp2, tmp2_bytes, it, flags, rows, count, (int)n, stream);
+  int32_t hcount = 0;
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&hcount, count, sizeof hcount, hipMemcpyDeviceToHost, stream);
   const hipError_t se = hipStreamSynchronize(stream);
+  *nrows = hcount;
+  (void)hipFree(tmp2);
   (void)hipFree(keys);
   (void)hipFree(keys2);
   (void)hipFree(ids);
@@ -1791,11 +2008,17 @@ hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, 
 }
 
 template <typename OffT>
-hipError_t launch_pull_rows(const OffT* off, long long n, long long m, long long waves,
-                            int32_t* out, hipStream_t stream) {
+hipError_t launch_pull_rows(const OffT* off, const int32_t* rows, long long nrows, long long m,
+                            long long waves, int32_t* out, hipStream_t stream) {
   const long long threads = waves + 1;
   hipLaunchKernelGGL(pull_rows_kernel<OffT>, dim3((unsigned)((threads + kBlock - 1) / kBlock)),
-                     dim3(kBlock), 0, stream, off, n, m, waves, out);
+                     dim3(kBlock), 0, stream, off, rows, nrows, m, waves, out);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_mbfs_depth(const MbfsArgs<OffT>& a, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(mbfs_depth_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -1831,11 +2054,13 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
                                              hipStream_t);                                      \
   template const void* mbfs_level_symbol<OffT>(bool);                                           \
   template hipError_t launch_mbfs_level<OffT>(const MbfsArgs<OffT>*, bool, int, hipStream_t);    \
-  template hipError_t launch_pull_rows<OffT>(const OffT*, long long, long long, long long,       \
-                                             int32_t*, hipStream_t);                             \
+  template hipError_t launch_pull_rows<OffT>(const OffT*, const int32_t*, long long, long long,  \
+                                             long long, int32_t*, hipStream_t);                  \
   template hipError_t build_degree_ranks<OffT>(const OffT*, const int32_t*, long long, long long, \
-                                               int32_t*, int32_t*, int32_t*, int, hipStream_t);  \
+                                               int32_t*, int32_t*, int32_t*, int32_t*,           \
+                                               long long*, int, hipStream_t);                    \
   template hipError_t launch_mbfs_stats<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
+  template hipError_t launch_mbfs_depth<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
   template hipError_t launch_word_weights<OffT>(const OffT*, long long, unsigned long long*, int, \
                                                hipStream_t);                                    \
   template hipError_t launch_graph_symmetry<OffT>(const OffT*, const int32_t*, long long,        \

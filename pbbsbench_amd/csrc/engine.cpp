@@ -42,14 +42,16 @@ template <typename OffT>
 hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
                              hipStream_t stream);
 template <typename OffT>
-hipError_t launch_pull_rows(const OffT* off, long long n, long long m, long long waves,
-                            int32_t* out, hipStream_t stream);
+hipError_t launch_pull_rows(const OffT* off, const int32_t* rows, long long nrows, long long m,
+                            long long waves, int32_t* out, hipStream_t stream);
 template <typename OffT>
 hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, long long m,
-                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int grid,
-                              hipStream_t stream);
+                              int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int32_t* rows,
+                              long long* nrows, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_mbfs_depth(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_word_weights(const OffT* off, long long n, unsigned long long* ww, int grid,
                                hipStream_t stream);
@@ -118,6 +120,7 @@ struct pbbs_ctx {
     int K = 0;
     int64_t launched = 0;
     int64_t limit = 0;
+    bool finalized = false;  // batched: the depth pass after the last level is queued
   } run;
   // batched BFS
   int64_t cap_bn = 0;                    // vertices the mask buffers hold
@@ -128,6 +131,7 @@ struct pbbs_ctx {
   int64_t cap_bout = 0;                  // int32 entries of each output buffer
   int32_t* b_parent = nullptr;
   int32_t* b_depth = nullptr;
+  uint8_t* b_dlev = nullptr;             // [cap_bn][64] per-vertex level table
   bool b_last_args_off64 = false;
   alignas(16) unsigned char b_last_args[sizeof(MbfsArgs<int64_t>)] = {};  // host copy for stats
   std::map<std::tuple<int, int, int>, hipGraphExec_t> batch_graphs;
@@ -151,6 +155,8 @@ struct pbbs_graph {
   int32_t* d_rank = nullptr;             // batched BFS: decreasing-degree rank of each vertex
   int32_t* d_inv_rank = nullptr;         // batched BFS: vertex of each rank
   int32_t* d_adj_rank = nullptr;         // batched BFS: adj in rank ids
+  int32_t* d_rows = nullptr;             // batched BFS: vertices of degree > 0, ascending
+  int64_t nrows = 0;
   int64_t isolated = 0;
   int64_t max_degree = 0;
   int64_t device_bytes = 0;
@@ -241,6 +247,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   for (auto& kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second);
   free_workspace(c);
   (void)hipFree(c->b_masks);
+  (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_state);
   (void)hipFree(c->b_args);
   (void)hipFree(c->b_sources);
@@ -375,6 +382,7 @@ int pbbs_graph_free(pbbs_graph* g) {
   (void)hipFree(g->d_rank);
   (void)hipFree(g->d_inv_rank);
   (void)hipFree(g->d_adj_rank);
+  (void)hipFree(g->d_rows);
   delete g;
   return PBBS_OK;
 }
@@ -516,8 +524,8 @@ static int ensure_pull_rows(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const int
   }
   int32_t* d = nullptr;
   HIP_TRY(hipMalloc(&d, (waves + 1) * sizeof(int32_t)));
-  hipError_t e = launch_pull_rows<OffT>(static_cast<const OffT*>(g->d_off), g->n, g->m, waves, d,
-                                        c->stream);
+  hipError_t e = launch_pull_rows<OffT>(static_cast<const OffT*>(g->d_off), g->d_rows, g->nrows,
+                                        g->m, waves, d, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) {
     (void)hipFree(d);
@@ -534,28 +542,35 @@ template <typename OffT>
 static int ensure_ranks(pbbs_ctx* c, pbbs_graph* g) {
   std::lock_guard<std::mutex> lk(g->meta_mu);
   if (g->d_rank) return PBBS_OK;
-  int32_t *rank = nullptr, *inv = nullptr, *adj_rank = nullptr;
-  if (hipMalloc(&rank, g->n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&inv, g->n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&adj_rank, std::max<int64_t>(g->m, 1) * sizeof(int32_t)) != hipSuccess) {
-    (void)hipFree(rank);
-    (void)hipFree(inv);
-    return fail(PBBS_ERR_NOMEM, "degree-rank arrays for n=%lld m=%lld failed", (long long)g->n,
-                (long long)g->m);
-  }
-  const hipError_t e =
-      build_degree_ranks<OffT>(static_cast<const OffT*>(g->d_off), g->d_adj, g->n, g->m, rank,
-                               inv, adj_rank, c->num_cus * 8, c->stream);
-  if (e != hipSuccess) {
+  int32_t *rank = nullptr, *inv = nullptr, *adj_rank = nullptr, *rows = nullptr;
+  auto release = [&]() {
     (void)hipFree(rank);
     (void)hipFree(inv);
     (void)hipFree(adj_rank);
+    (void)hipFree(rows);
+  };
+  if (hipMalloc(&rank, g->n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&inv, g->n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&rows, g->n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&adj_rank, std::max<int64_t>(g->m, 1) * sizeof(int32_t)) != hipSuccess) {
+    release();
+    return fail(PBBS_ERR_NOMEM, "degree-rank arrays for n=%lld m=%lld failed", (long long)g->n,
+                (long long)g->m);
+  }
+  long long nrows = 0;
+  const hipError_t e =
+      build_degree_ranks<OffT>(static_cast<const OffT*>(g->d_off), g->d_adj, g->n, g->m, rank,
+                               inv, adj_rank, rows, &nrows, c->num_cus * 8, c->stream);
+  if (e != hipSuccess) {
+    release();
     return fail(PBBS_ERR_HIP, "degree ranks: %s", hipGetErrorString(e));
   }
   g->d_rank = rank;
   g->d_inv_rank = inv;
   g->d_adj_rank = adj_rank;
-  g->device_bytes += (2 * g->n + g->m) * (int64_t)sizeof(int32_t);
+  g->d_rows = rows;
+  g->nrows = nrows;
+  g->device_bytes += (3 * g->n + g->m) * (int64_t)sizeof(int32_t);
   return PBBS_OK;
 }
 
@@ -649,6 +664,21 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
     if (e != hipSuccess) return fail(PBBS_ERR_HIP, "event query: %s", hipGetErrorString(e));
   }
   if (c->host_status[0]) {
+    pbbs_ctx::Run& r = c->run;
+    if (r.kind == 1 && !r.finalized) {
+      // batched: write the depth outputs, then report done once that is through
+      const int grid = std::max(1, c->num_cus * 8);
+      if (r.off64)
+        HIP_TRY(launch_mbfs_depth<int64_t>(
+            *reinterpret_cast<MbfsArgs<int64_t>*>(c->b_last_args), grid, c->stream));
+      else
+        HIP_TRY(launch_mbfs_depth<uint32_t>(
+            *reinterpret_cast<MbfsArgs<uint32_t>*>(c->b_last_args), grid, c->stream));
+      HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+      r.finalized = true;
+      if (!wait) return PBBS_OK;
+      HIP_TRY(hipEventSynchronize(c->ev_end));
+    }
     *done = true;
     return PBBS_OK;
   }
@@ -676,6 +706,7 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.grid = grid;
   r.K = c->tuning.chunk_levels;
   r.launched = 0;
+  r.finalized = false;
   r.limit = g->n + 2;
   if (c->tuning.use_graphs) {  // capture (first use only) while the stream is idle
     hipGraphExec_t exec = nullptr;
@@ -856,8 +887,11 @@ static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, b
     (void)hipFree(c->b_masks);
     c->b_masks = nullptr;
     c->cap_bn = 0;
+    (void)hipFree(c->b_dlev);
+    c->b_dlev = nullptr;
     if (hipMalloc(&c->b_masks, (5 * n + 4 * ((n + kWave - 1) / kWave)) *
-                                   sizeof(unsigned long long)) != hipSuccess)
+                                   sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc(&c->b_dlev, n * kMaxSources) != hipSuccess)
       return fail(PBBS_ERR_NOMEM, "batched mask buffers for n=%lld failed", (long long)n);
     c->cap_bn = n;
   }
@@ -925,6 +959,7 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.parent = d_parent;
   a.depth = d_depth;
   a.vis = c->b_masks;
+  a.dlev = c->b_dlev;
   for (int j = 0; j < 4; ++j) a.fr[j] = c->b_masks + (j + 1) * n;
   for (int j = 0; j < 4; ++j) a.fbm[j] = c->b_masks + 5 * n + j * ((n + kWave - 1) / kWave);
   a.q[0] = c->w_q;
@@ -950,16 +985,19 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   r.grid = batch_grid(c, g->off64, g->sorted);
   {
     const int32_t* rows = nullptr;
-    int rc = ensure_pull_rows<OffT>(c, g, (int64_t)r.grid * kWavesPerBlock, &rows);
-    if (rc == PBBS_OK) rc = ensure_ranks<OffT>(c, g);
+    int rc = ensure_ranks<OffT>(c, g);
+    if (rc == PBBS_OK) rc = ensure_pull_rows<OffT>(c, g, (int64_t)r.grid * kWavesPerBlock, &rows);
     if (rc != PBBS_OK) return rc;
     a.pull_rows = rows;
+    a.rows = g->d_rows;
+    a.nrows = g->nrows;
     a.rank = g->d_rank;
     a.inv_rank = g->d_inv_rank;
     a.adj_rank = g->d_adj_rank;
   }
   r.K = c->tuning.chunk_levels;
   r.launched = 0;
+  r.finalized = false;
   r.limit = n + 2;
   const int setup_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((n + kBlock - 1) / kBlock, c->num_cus * 16));
@@ -1066,7 +1104,7 @@ namespace pbbs {
 int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
   const int64_t nwords = (n + kWave - 1) / kWave;
   int64_t b = 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * n * 8;
-  if (nsrc > 0) b += 5 * n * 8 + 4 * nwords * 8 + 2 * (int64_t)nsrc * n * 4;
+  if (nsrc > 0) b += 5 * n * 8 + 4 * nwords * 8 + n * kMaxSources + 2 * (int64_t)nsrc * n * 4;
   return b;
 }
 

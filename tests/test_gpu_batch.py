@@ -129,3 +129,15 @@ def test_batch_hub_rows_split_across_wavefronts(grid):
         with pb.Context(0) as c:
             c.set_tuning(grid_blocks=grid, batch_pull_div=pull)
             check_batch(c, g, src)
+
+
+@pytest.mark.parametrize("pull", [0, 10, 1_000_000])
+def test_batch_deep_levels(pull):
+    """A 700-vertex path: depths reach 699, past the one-byte level table
+    (levels from 255 on are written straight to the outputs)."""
+    n = 700
+    u = np.arange(n - 1, dtype=np.int32)
+    g = pb.from_edges(n, u, u + 1)
+    with pb.Context(0) as c:
+        c.set_tuning(batch_pull_div=pull)
+        check_batch(c, g, [0, 350, 699, 5, 0])
