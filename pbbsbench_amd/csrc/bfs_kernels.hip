@@ -945,6 +945,36 @@ __device__ __forceinline__ unsigned long long seg_prev(unsigned long long x, int
   return k2 == key ? (((unsigned long long)hi << 32) | lo) : 0ull;
 }
 
+// Segmented inclusive OR where each lane knows `dist`, its distance to the
+// first lane of its segment (segments are runs of lanes).  Row-local steps
+// read 0 from lanes outside the 16-lane row (bound_ctrl), the two broadcast
+// steps bring in the previous rows' last lanes.
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ __forceinline__ unsigned dpp_take(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, kBound);
+}
+__device__ __forceinline__ unsigned long long seg_or_scan_d(unsigned long long x, int dist,
+                                                            int lane16, int lane32) {
+  unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+#define PBBS_SEG_STEP(ctrl, rm, bound, cond)                        \
+  {                                                                 \
+    const unsigned l2 = dpp_take<ctrl, rm, bound>(lo);             \
+    const unsigned h2 = dpp_take<ctrl, rm, bound>(hi);             \
+    if (cond) {                                                     \
+      lo |= l2;                                                     \
+      hi |= h2;                                                     \
+    }                                                               \
+  }
+  PBBS_SEG_STEP(kRowShr1, 0xf, true, dist >= 1)
+  PBBS_SEG_STEP(kRowShr2, 0xf, true, dist >= 2)
+  PBBS_SEG_STEP(kRowShr4, 0xf, true, dist >= 4)
+  PBBS_SEG_STEP(kRowShr8, 0xf, true, dist >= 8)
+  PBBS_SEG_STEP(kRowBcast15, 0xa, false, dist > lane16)
+  PBBS_SEG_STEP(kRowBcast31, 0xc, false, dist > lane32)
+#undef PBBS_SEG_STEP
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 // Inclusive sum over the wavefront through DPP.
 __device__ __forceinline__ unsigned wave_inclusive_sum_dpp(unsigned x) {
   x += dpp_u32<kRowShr1, 0xf>(x, 0u);
@@ -1047,37 +1077,51 @@ __device__ void scan_window(const MbfsView<OffT>& a, gptr<const unsigned long lo
       if (!__ballot(h[s] != 0ull)) continue;   // no hit in this group: nothing changes
       const int p = cur.rowp[s];
       const int ps = p < 0 ? 0 : p;
+      // row p's lanes in this group: [first, last]; dist = lane - first
+      const int g0 = s * kWave;
+      const int ep = (int)__shfl(cur.excl, ps);
+      const int first = max(ep - g0, 0);
+      const int dist = p < 0 ? 0 : lane - first;
       // inclusive OR of h over the lanes of this group holding row p
-      const unsigned long long x = seg_or_scan(h[s], p);
-      const int vp = __shfl(r.v, ps);
-      const int sp = __shfl((int)r.split, ps);
+      const unsigned long long x = seg_or_scan_d(h[s], dist, lane & 15, lane & 31);
       unsigned long long nb;
-      if (kSorted)
-        nb = h[s] & ~seg_prev(x, p) & ~shfl64(r.found, ps);
-      else
+      if (kSorted) {
+        // exclusive part: the lane below, if it holds the same row
+        const unsigned plo = dpp_take<kWaveShr1, 0xf, true>((unsigned)x);
+        const unsigned phi = dpp_take<kWaveShr1, 0xf, true>((unsigned)(x >> 32));
+        const unsigned long long pre = dist >= 1 ? (((unsigned long long)phi << 32) | plo) : 0ull;
+        nb = h[s] & ~pre & ~shfl64(r.found, ps);
+      } else {
         nb = h[s];
+      }
+      const int sp = __shfl((int)r.split, ps);
+      const int vp = __shfl(r.v, ps);   // shuffles stay outside divergent code
       if (nb) {
         const int u = a.inv_rank[cur.ur[s]];
-        for (unsigned long long b = nb; b; b &= b - 1) {
-          const int i = __builtin_ctzll(b);
-          if (sp) {
-            const long long o = (long long)i * a.n + vp;
-            atomicMax(gen(&a.parent[o]), u);
+        if (sp) {
+          for (unsigned long long b = nb; b; b &= b - 1) {
+            const int i = __builtin_ctzll(b);
+            atomicMax(gen(&a.parent[(long long)i * a.n + vp]), u);
             put_depth(a, i, vp, d);
-          } else if (kSorted) {
-            tile[p * kTileStride + i] = u;
-          } else {
-            atomicMax(&tile[p * kTileStride + i], u);
+          }
+        } else {
+          int32_t* trow = tile + p * kTileStride;
+          for (unsigned b = (unsigned)nb; b; b &= b - 1) {
+            if (kSorted) trow[__builtin_ctz(b)] = u;
+            else atomicMax(&trow[__builtin_ctz(b)], u);
+          }
+          for (unsigned b = (unsigned)(nb >> 32); b; b &= b - 1) {
+            if (kSorted) trow[32 + __builtin_ctz(b)] = u;
+            else atomicMax(&trow[32 + __builtin_ctz(b)], u);
           }
         }
       }
       // the last lane of row q's segment in this group holds the row's OR
-      const long long g0 = (long long)s * kWave;
-      const long long f = max((long long)cur.excl, g0);
-      const long long e = min((long long)cur.excl + cur.cnt, g0 + kWave);
-      const int src = e > f ? (int)(e - 1 - g0) : lane;
+      const int qf = max((int)cur.excl - g0, 0);
+      const int qe = min((int)(cur.excl + cur.cnt) - g0, kWave);
+      const int src = qe > qf ? qe - 1 : lane;
       const unsigned long long t = shfl64(x, src);
-      if (e > f) r.found |= t;
+      if (qe > qf) r.found |= t;
     }
     if (nxt.total == 0) break;
     scan_consume(r, nxt);
