@@ -249,6 +249,22 @@ int pbbs_gen_randlocal(int64_t n, int32_t k, uint64_t seed, pbbs_csr* out);
 int pbbs_csr_from_edges(int64_t n, int64_t num_edges, const int32_t* u,
                         const int32_t* v, int symmetrise, pbbs_csr* out);
 
+/* ---- graphs built in HBM ----------------------------------------------- */
+/* The generators below, run on the GPU: the resulting graph is bit-identical
+ * to pbbs_gen_rmat / pbbs_gen_grid3d (same arguments) followed by
+ * pbbs_graph_upload, without the host build or the copy.  flags: 0 (uint32
+ * offsets) or PBBS_GRAPH_OFFSETS_I64.  Replaces running rMatGraph
+ * (testData/graphData/rMatGraph.C:33-79) / gridGraph (gridGraph.C:53-72) and
+ * reading the file back (common/graphIO.h:191-215). */
+int pbbs_graph_gen_rmat(pbbs_ctx* ctx, int32_t scale, int64_t draws, double a, double b, double c,
+                        uint64_t seed, uint64_t permute_seed, uint32_t flags, pbbs_graph** out);
+int pbbs_graph_gen_grid3d(pbbs_ctx* ctx, int32_t d, uint32_t flags, pbbs_graph** out);
+/* Copies a resident graph back into a host CSR (int64 offsets; free it with
+ * pbbs_csr_free).  Replaces writeGraphToFile's source of data
+ * (common/graphIO.h:78-105). */
+int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out);
+
+
 /* PBBS AdjacencyGraph text format (docs/fileFormats/graph.md:16-39;
  * reader common/graphIO.h:191-215, writer common/graphIO.h:78-105).  Rows
  * are kept in file order. */

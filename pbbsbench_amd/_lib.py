@@ -131,6 +131,11 @@ SIGNATURES = [
     ("pbbs_bfs_profile", C.c_int,
      [_P, _P, C.c_int32, C.POINTER(LevelProfile), C.c_int32, C.POINTER(C.c_int32),
       C.POINTER(C.c_double)]),
+    ("pbbs_graph_gen_rmat", C.c_int,
+     [_P, C.c_int32, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint64,
+      C.c_uint32, C.POINTER(_P)]),
+    ("pbbs_graph_gen_grid3d", C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
+    ("pbbs_graph_download", C.c_int, [_P, C.POINTER(Csr)]),
     ("pbbs_server_create", C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
     ("pbbs_server_destroy", C.c_int, [_P]),
     ("pbbs_server_add_graph", C.c_int,

@@ -207,6 +207,30 @@ class Context:
         return DeviceGraph(self, h, g.n, g.m)
 
 
+    def gen_rmat(self, scale, draws, a, b, c, seed=1, permute_seed=0, wide_offsets=False):
+        """rmat(...) built on the GPU, straight into HBM; the same graph bit for
+        bit as ctx.upload(rmat(...)) (testData/graphData/rMatGraph.C:33-79)."""
+        h = C.c_void_p()
+        check(lib().pbbs_graph_gen_rmat(self._h, scale, draws, a, b, c, seed, permute_seed,
+                                        _lib.PBBS_GRAPH_OFFSETS_I64 if wide_offsets else 0,
+                                        C.byref(h)), "graph_gen_rmat")
+        g = DeviceGraph(self, h, 0, 0)
+        i = g.info
+        g.n, g.m = i["n"], i["m"]
+        return g
+
+    def gen_grid3d(self, d, wide_offsets=False):
+        """grid3d(d) built on the GPU (testData/graphData/gridGraph.C:53-72)."""
+        h = C.c_void_p()
+        check(lib().pbbs_graph_gen_grid3d(self._h, d,
+                                          _lib.PBBS_GRAPH_OFFSETS_I64 if wide_offsets else 0,
+                                          C.byref(h)), "graph_gen_grid3d")
+        g = DeviceGraph(self, h, 0, 0)
+        i = g.info
+        g.n, g.m = i["n"], i["m"]
+        return g
+
+
 class DeviceGraph:
     """A CSR resident in HBM."""
 
@@ -229,6 +253,12 @@ class DeviceGraph:
 
     def numVertices(self):
         return self.n
+
+    def download(self):
+        """The resident CSR as a HostGraph (int64 offsets)."""
+        c = _lib.Csr()
+        check(lib().pbbs_graph_download(self._h, C.byref(c)), "graph_download")
+        return HostGraph._take(c)
 
     @property
     def info(self):

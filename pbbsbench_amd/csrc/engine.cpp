@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -1109,6 +1110,8 @@ int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
 }
 
 hipStream_t ctx_stream(pbbs_ctx* c) { return c->stream; }
+int ctx_device(const pbbs_ctx* c) { return c->device; }
+int ctx_num_cus(const pbbs_ctx* c) { return c->num_cus; }
 int64_t graph_n(const pbbs_graph* g) { return g->n; }
 bool graph_symmetric(const pbbs_graph* g) { return g->symmetric; }
 
@@ -1154,3 +1157,41 @@ int async_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
 }
 
 }  // namespace pbbs
+
+// ------------------------------------------------------------ graph download
+extern "C" int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out) {
+  clear_error();
+  if (!g || !out) return fail(PBBS_ERR_INVALID, "NULL argument");
+  out->n = out->m = 0;
+  out->offsets = nullptr;
+  out->adj = nullptr;
+  HIP_TRY(hipSetDevice(g->ctx->device));
+  HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  int64_t* off = static_cast<int64_t*>(std::malloc((g->n + 1) * sizeof(int64_t)));
+  int32_t* adj = static_cast<int32_t*>(std::malloc(std::max<int64_t>(g->m, 1) * sizeof(int32_t)));
+  if (!off || !adj) {
+    std::free(off);
+    std::free(adj);
+    return fail(PBBS_ERR_NOMEM, "host CSR allocation failed");
+  }
+  hipError_t e = hipSuccess;
+  if (g->off64) {
+    e = hipMemcpy(off, g->d_off, (g->n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost);
+  } else {
+    std::vector<uint32_t> o32(g->n + 1);
+    e = hipMemcpy(o32.data(), g->d_off, (g->n + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    for (int64_t i = 0; i <= g->n; ++i) off[i] = o32[i];
+  }
+  if (e == hipSuccess && g->m > 0)
+    e = hipMemcpy(adj, g->d_adj, g->m * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    std::free(off);
+    std::free(adj);
+    return fail(PBBS_ERR_HIP, "graph download: %s", hipGetErrorString(e));
+  }
+  out->n = g->n;
+  out->m = g->m;
+  out->offsets = off;
+  out->adj = adj;
+  return PBBS_OK;
+}

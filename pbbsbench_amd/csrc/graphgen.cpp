@@ -148,13 +148,7 @@ int csr_from_keys(int64_t n, Keys& keys, int vb, pbbs_csr* out) {
 // (common/graphUtils.h:166-189).
 void permute_keys(Keys& keys, int64_t n, int vb, uint64_t seed) {
   std::vector<int32_t> perm(n);
-  for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
-  uint64_t s = mix64(seed ^ 0x5DEECE66Dull);
-  for (int64_t i = n - 1; i > 0; --i) {
-    s = mix64(s);
-    const int64_t j = (int64_t)(s % (uint64_t)(i + 1));
-    std::swap(perm[i], perm[j]);
-  }
+  pbbs::make_permutation(n, seed, perm.data());
   const uint64_t vmask = (uint64_t(1) << vb) - 1;
 #pragma omp parallel for schedule(static)
   for (int64_t i = 0; i < keys.size(); ++i) {
@@ -165,6 +159,20 @@ void permute_keys(Keys& keys, int64_t n, int vb, uint64_t seed) {
 }
 
 }  // namespace
+
+namespace pbbs {
+
+void make_permutation(int64_t n, uint64_t seed, int32_t* perm) {
+  for (int64_t i = 0; i < n; ++i) perm[i] = (int32_t)i;
+  uint64_t s = mix64(seed ^ 0x5DEECE66Dull);
+  for (int64_t i = n - 1; i > 0; --i) {
+    s = mix64(s);
+    const int64_t j = (int64_t)(s % (uint64_t)(i + 1));
+    std::swap(perm[i], perm[j]);
+  }
+}
+
+}  // namespace pbbs
 
 extern "C" {
 
