@@ -34,6 +34,15 @@ METRIC = ("GTEPS + % HBM roofline (1 GPU); batched-64-source vs sequential GTEPS
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
 
+# GPU builders (ctx, wide) -> DeviceGraph; the same graphs as the host builders,
+# bit for bit (tests/test_gpu_graphgen.py), built in HBM in milliseconds.
+GPU_BUILDERS = {
+    "rmat24": lambda ctx, w: ctx.gen_rmat(24, 2 ** 28, 0.57, 0.19, 0.19, seed=1, wide_offsets=w),
+    "rmat16": lambda ctx, w: ctx.gen_rmat(16, 2 ** 19, 0.5, 0.1, 0.1, seed=1, wide_offsets=w),
+    "torus216": lambda ctx, w: ctx.gen_grid3d(216, wide_offsets=w),
+    "rmat22": lambda ctx, w: ctx.gen_rmat(22, 2 ** 26, 0.57, 0.19, 0.19, seed=1, wide_offsets=w),
+}
+
 WORKLOADS = {
     # name: (builder, wide offsets, description)
     "rmat24": (lambda: __import__("pbbsbench_amd").rmat(24, 2 ** 28, 0.57, 0.19, 0.19, seed=1),
@@ -92,6 +101,8 @@ def main():
     ap.add_argument("--qps", type=float, default=0.0,
                     help="serve: Poisson arrival rate (0 = half the measured burst peak)")
     ap.add_argument("--queries", type=int, default=2000, help="serve: queries in the stream")
+    ap.add_argument("--host-gen", action="store_true",
+                    help="build the graphs with the host generators (default: on the GPU)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
@@ -106,12 +117,6 @@ def main():
     import pbbsbench_amd as pb
 
     build, wide, desc = WORKLOADS[args.workload]
-    t0 = time.time()
-    g = build()
-    gen_s = time.time() - t0
-    src = pick_source(g, args.workload, args.seed)
-    deg = g.degrees()
-
     ctx = pb.Context(0)
     tune = {}
     if args.td_to_bu_div is not None:
@@ -121,8 +126,21 @@ def main():
     if tune:
         ctx.set_tuning(**tune)
     t0 = time.time()
-    dg = ctx.upload(g, wide_offsets=wide)
-    upload_s = time.time() - t0
+    if args.workload in GPU_BUILDERS and not args.host_gen:
+        dg = GPU_BUILDERS[args.workload](ctx, wide)   # built in HBM
+        gen_where = "gpu"
+        gen_s = time.time() - t0
+        g = dg.download()   # host copy for source choice, CPU baseline and check
+        upload_s = 0.0
+    else:
+        g = build()
+        gen_where = "host"
+        gen_s = time.time() - t0
+        t0 = time.time()
+        dg = ctx.upload(g, wide_offsets=wide)
+        upload_s = time.time() - t0
+    src = pick_source(g, args.workload, args.seed)
+    deg = g.degrees()
 
     for _ in range(args.warmup):
         dg.bfs_device(src)
@@ -164,7 +182,7 @@ def main():
             "event_ms_per_bfs": round(event_ms, 4),
             "gteps_graph500": round(gteps / 2, 2),
             "d2h_ms": round(st_host["copy_ms"], 3),
-            "gen_s": round(gen_s, 1), "upload_s": round(upload_s, 2),
+            "gen_s": round(gen_s, 3), "gen": gen_where, "upload_s": round(upload_s, 2),
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -221,14 +239,19 @@ def bench_batch(args):
     on RMAT 2^22 (2^26 draws, a=.57 b=.19 c=.19), uint32 offsets.  A step is
     one batched BFS of all 64 sources with outputs [64][n] written to HBM."""
     import pbbsbench_amd as pb
+    ctx = pb.Context(0)
     t0 = time.time()
-    g = pb.rmat(22, 2 ** 26, 0.57, 0.19, 0.19, seed=1)
-    gen_s = time.time() - t0
+    if args.host_gen:
+        g = pb.rmat(22, 2 ** 26, 0.57, 0.19, 0.19, seed=1)
+        gen_s = time.time() - t0
+        dg = ctx.upload(g)
+    else:
+        dg = GPU_BUILDERS["rmat22"](ctx, False)
+        gen_s = time.time() - t0
+        g = dg.download()
     deg = g.degrees()
     rng = np.random.default_rng(args.seed)
     src = rng.choice(np.flatnonzero(deg > 0), 64, replace=False).astype(np.int32)
-    ctx = pb.Context(0)
-    dg = ctx.upload(g)
     for _ in range(max(args.warmup, 1)):
         dg.bfs_batch_device(src)
         dg.bfs_device(int(src[0]))
@@ -281,7 +304,8 @@ def bench_batch(args):
             "sequential_wall_ms_64": round(seq_wall, 3),
             "sequential_gteps": round(seq_edges / (seq_wall / 1e3) / 1e9, 2),
             "batch_vs_sequential": round(seq_wall / wall, 3),
-            "gen_s": round(gen_s, 1), "parallelism": "single GPU",
+            "gen_s": round(gen_s, 3), "gen": "host" if args.host_gen else "gpu",
+            "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",

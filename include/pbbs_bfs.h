@@ -151,6 +151,40 @@ int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
 int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
                    int32_t* parents, int32_t* depths, uint32_t out_flags, pbbs_bfs_stats* stats);
 
+/* Device pointers of the context's own output buffers: those pbbs_bfs
+ * (batched = 0; n entries each) or pbbs_bfs_batch (batched = 1; [k][n]) last
+ * wrote when the caller passed no output pointers with PBBS_OUT_DEVICE.
+ * Valid until the next call of that kind on the context.  No reference
+ * counterpart (for callers that consume the outputs in HBM). */
+int pbbs_ctx_outputs(const pbbs_ctx* ctx, int batched, const int32_t** parent,
+                     const int32_t** depth);
+
+/* ---- checker ----------------------------------------------------------- */
+/* Validates a BFS output on the GPU: bench/BFSCheck.C:59-90 (with the source
+ * as a parameter; levels from the parents by level_number, BFSCheck.C:39-56),
+ * plus the max-parent rule of deterministicBFS and, if depths are given,
+ * depth == level.  parent/depth are host arrays of n entries, or device
+ * pointers with PBBS_CHECK_DEVICE_INPUT (e.g. one row of a batched output).
+ * Replaces running bench/BFSCheck on the written files. */
+#define PBBS_CHECK_DEVICE_INPUT 0x1u
+
+typedef struct {
+    int32_t valid;         /* 1 iff BFSCheck's rule accepts every vertex */
+    int32_t cycle;         /* the parents contain a cycle (no level fixed point) */
+    int64_t first_bad;     /* first vertex the rule rejects (or with a parent outside
+                              [-1, n)), -1 if none */
+    int64_t bad;           /* vertices rejected */
+    int64_t max_rule_bad;  /* vertices whose parent is not the largest neighbour one
+                              level up (-1 if unreached, itself for the source) */
+    int64_t depth_bad;     /* vertices whose depth differs from their level */
+    int64_t reached;       /* vertices with a level below n */
+    int32_t max_level;
+    int32_t pad;
+} pbbs_check_result;
+
+int pbbs_check_bfs(pbbs_ctx* ctx, const pbbs_graph* g, int32_t source, const int32_t* parent,
+                   const int32_t* depth, uint32_t flags, pbbs_check_result* out);
+
 /* ---- serving loop ------------------------------------------------------ */
 /* Graphs stay resident; queries arrive from any host thread and complete
  * asynchronously.  A worker thread keeps up to `slots` runs in flight (each

@@ -111,6 +111,26 @@ class ServerResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+PBBS_CHECK_DEVICE_INPUT = 0x1
+
+
+class CheckResult(C.Structure):
+    _fields_ = [
+        ("valid", C.c_int32),
+        ("cycle", C.c_int32),
+        ("first_bad", C.c_int64),
+        ("bad", C.c_int64),
+        ("max_rule_bad", C.c_int64),
+        ("depth_bad", C.c_int64),
+        ("reached", C.c_int64),
+        ("max_level", C.c_int32),
+        ("pad", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 # (name, restype, argtypes) for every symbol include/pbbs_bfs.h declares.
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
@@ -136,6 +156,9 @@ SIGNATURES = [
       C.c_uint32, C.POINTER(_P)]),
     ("pbbs_graph_gen_grid3d", C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
     ("pbbs_graph_download", C.c_int, [_P, C.POINTER(Csr)]),
+    ("pbbs_ctx_outputs", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P)]),
+    ("pbbs_check_bfs", C.c_int,
+     [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(CheckResult)]),
     ("pbbs_server_create", C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
     ("pbbs_server_destroy", C.c_int, [_P]),
     ("pbbs_server_add_graph", C.c_int,

@@ -187,6 +187,14 @@ class Context:
             setattr(t, k, v)
         check(lib().pbbs_tuning_set(self._h, C.byref(t)), "tuning_set")
 
+    def outputs(self, batched=False):
+        """Device addresses (ints) of the context's own parent and depth buffers,
+        as the last BFS (or batched BFS) with device outputs left them."""
+        p, d = C.c_void_p(), C.c_void_p()
+        check(lib().pbbs_ctx_outputs(self._h, 1 if batched else 0, C.byref(p), C.byref(d)),
+              "ctx_outputs")
+        return p.value, d.value
+
     def upload(self, g, wide_offsets=False, assume_symmetric=False):
         """Copies a HostGraph into HBM.  wide_offsets keeps int64 offsets;
         otherwise they are narrowed to uint32 (the reference's edgeId)."""
@@ -253,6 +261,30 @@ class DeviceGraph:
 
     def numVertices(self):
         return self.n
+
+    def check(self, source, parent, depth=None):
+        """GPU checker (bench/BFSCheck.C:59-90 + the max-parent rule): returns
+        {valid, cycle, first_bad, bad, max_rule_bad, depth_bad, reached, max_level}."""
+        parent = np.ascontiguousarray(parent, dtype=np.int32)
+        if parent.size != self.n:
+            raise ValueError("parent must hold n entries")
+        dep = None
+        if depth is not None:
+            dep = np.ascontiguousarray(depth, dtype=np.int32)
+            if dep.size != self.n:
+                raise ValueError("depth must hold n entries")
+        r = _lib.CheckResult()
+        check(lib().pbbs_check_bfs(self.ctx._h, self._h, int(source), _ptr(parent),
+                                   _ptr(dep) if dep is not None else None, 0, C.byref(r)),
+              "check_bfs")
+        return r.as_dict()
+
+    def check_device(self, source, parent_ptr, depth_ptr=None):
+        """check() on outputs already in HBM (device pointers)."""
+        r = _lib.CheckResult()
+        check(lib().pbbs_check_bfs(self.ctx._h, self._h, int(source), parent_ptr, depth_ptr,
+                                   _lib.PBBS_CHECK_DEVICE_INPUT, C.byref(r)), "check_bfs")
+        return r.as_dict()
 
     def download(self):
         """The resident CSR as a HostGraph (int64 offsets)."""

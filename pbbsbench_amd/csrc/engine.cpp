@@ -1111,6 +1111,11 @@ int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
 
 hipStream_t ctx_stream(pbbs_ctx* c) { return c->stream; }
 int ctx_device(const pbbs_ctx* c) { return c->device; }
+const void* graph_dev_off(const pbbs_graph* g) { return g->d_off; }
+const int32_t* graph_dev_adj(const pbbs_graph* g) { return g->d_adj; }
+bool graph_off64(const pbbs_graph* g) { return g->off64; }
+int64_t graph_m(const pbbs_graph* g) { return g->m; }
+int graph_device(const pbbs_graph* g) { return g->ctx->device; }
 int ctx_num_cus(const pbbs_ctx* c) { return c->num_cus; }
 int64_t graph_n(const pbbs_graph* g) { return g->n; }
 bool graph_symmetric(const pbbs_graph* g) { return g->symmetric; }
@@ -1193,5 +1198,15 @@ extern "C" int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out) {
   out->m = g->m;
   out->offsets = off;
   out->adj = adj;
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_ctx_outputs(const pbbs_ctx* c, int batched, const int32_t** parent,
+                                const int32_t** depth) {
+  clear_error();
+  if (!c || !parent || !depth) return fail(PBBS_ERR_INVALID, "NULL argument");
+  *parent = batched ? c->b_parent : c->w_parent;
+  *depth = batched ? c->b_depth : c->w_depth;
+  if (!*parent) return fail(PBBS_ERR_STATE, "no %s output buffers yet", batched ? "batched" : "BFS");
   return PBBS_OK;
 }
