@@ -305,6 +305,23 @@ int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out);
 int pbbs_read_adjacency_graph(const char* path, pbbs_csr* out);
 int pbbs_write_adjacency_graph(const char* path, const pbbs_csr* g);
 
+/* Binary CSR file, this library's own format (the reference has no binary
+ * graph file): 32-byte header (magic "PBBSCSR1", n u64, m u64, 8 reserved
+ * zero bytes), then offsets int64[n+1], then adjacency int32[m], all
+ * little-endian.  Reading it is a memory copy, so a graph parsed once from
+ * text reloads at disk speed.  The reader rejects a size that does not match
+ * its header. */
+int pbbs_write_graph_binary(const char* path, const pbbs_csr* g);
+int pbbs_read_graph_binary(const char* path, pbbs_csr* out);
+
+/* Reads a graph file (AdjacencyGraph text, or the binary format above,
+ * told apart by the magic) into pinned host staging and uploads it as
+ * pbbs_graph_upload would (flags: PBBS_GRAPH_OFFSETS_I64,
+ * PBBS_GRAPH_ASSUME_SYMMETRIC).  Replaces the benchmark driver's
+ * readGraphFromFile + graph-to-device step (common/graphIO.h:191-215,
+ * benchmarks/breadthFirstSearch/bench/BFSTime.C:57-58). */
+int pbbs_graph_read(pbbs_ctx* ctx, const char* path, uint32_t flags, pbbs_graph** out);
+
 /* PBBS sequenceInt text format (docs/fileFormats/sequence.md:10-26; writer
  * common/IO.h:151-166 via common/sequenceIO.h:189-193, reader
  * common/IO.h:217-229).  The reader allocates *out with malloc. */

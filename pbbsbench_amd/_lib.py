@@ -179,6 +179,9 @@ SIGNATURES = [
      [C.c_int64, C.c_int64, _P, _P, C.c_int, C.POINTER(Csr)]),
     ("pbbs_read_adjacency_graph", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     ("pbbs_write_adjacency_graph", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    ("pbbs_write_graph_binary", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    ("pbbs_read_graph_binary", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
+    ("pbbs_graph_read", C.c_int, [_P, C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     ("pbbs_write_sequence_int", C.c_int, [C.c_char_p, _P, C.c_int64]),
     ("pbbs_read_sequence_int", C.c_int, [C.c_char_p, C.POINTER(_I32P), C.POINTER(C.c_int64)]),
     ("pbbs_free", None, [_P]),

@@ -128,6 +128,20 @@ def write_graph_to_file(g, path):
     check(lib().pbbs_write_adjacency_graph(str(path).encode(), C.byref(c)), "writeGraphToFile")
 
 
+def write_graph_binary(g, path):
+    """Writes the binary CSR file ("PBBSCSR1" header, int64 offsets, int32
+    adjacency); no reference counterpart (the reference has text formats only)."""
+    c = g._as_csr()
+    check(lib().pbbs_write_graph_binary(str(path).encode(), C.byref(c)), "write_graph_binary")
+
+
+def read_graph_binary(path):
+    """Reads a file written by write_graph_binary."""
+    out = _lib.Csr()
+    check(lib().pbbs_read_graph_binary(str(path).encode(), C.byref(out)), "read_graph_binary")
+    return HostGraph._take(out)
+
+
 def write_sequence_to_file(a, path):
     """sequenceInt writer (common/sequenceIO.h:189-193, common/IO.h:151-166)."""
     a = np.ascontiguousarray(a, dtype=np.int32)
@@ -214,6 +228,21 @@ class Context:
                                       C.byref(h)), "pbbs_graph_upload")
         return DeviceGraph(self, h, g.n, g.m)
 
+
+    def read_graph(self, path, wide_offsets=False, assume_symmetric=False):
+        """Reads an AdjacencyGraph text file or a binary CSR file (told apart
+        by its magic) straight into HBM through pinned staging: the
+        readGraphFromFile + upload pair (common/graphIO.h:191-215)."""
+        flags = _lib.PBBS_GRAPH_OFFSETS_I64 if wide_offsets else 0
+        if assume_symmetric:
+            flags |= _lib.PBBS_GRAPH_ASSUME_SYMMETRIC
+        h = C.c_void_p()
+        check(lib().pbbs_graph_read(self._h, str(path).encode(), flags, C.byref(h)),
+              "graph_read")
+        g = DeviceGraph(self, h, 0, 0)
+        i = g.info
+        g.n, g.m = i["n"], i["m"]
+        return g
 
     def gen_rmat(self, scale, draws, a, b, c, seed=1, permute_seed=0, wide_offsets=False):
         """rmat(...) built on the GPU, straight into HBM; the same graph bit for

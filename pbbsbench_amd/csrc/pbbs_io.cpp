@@ -100,28 +100,43 @@ std::string header_of(const std::vector<char>& buf, int64_t* after) {
   return std::string(buf.data() + p, buf.data() + q);
 }
 
+// Appends "%lld\n" for x at p; returns the end.  Digits are produced in
+// reverse into a small buffer, then copied.
+inline char* put_line(char* p, long long x) {
+  unsigned long long u = x < 0 ? 0ull - (unsigned long long)x : (unsigned long long)x;
+  if (x < 0) *p++ = '-';
+  char tmp[24];
+  int k = 0;
+  do {
+    tmp[k++] = char('0' + u % 10);
+    u /= 10;
+  } while (u);
+  while (k) *p++ = tmp[--k];
+  *p++ = '\n';
+  return p;
+}
+
 // Formats a[0..n) as "%lld\n" lines in parallel blocks and writes them.
 template <typename T>
 int write_lines(FILE* f, const T* a, int64_t n) {
   const int64_t block = 1 << 20;
   const int64_t nb = (n + block - 1) / block;
   const int T_ = std::max(1, omp_get_max_threads());
+  std::vector<std::vector<char>> out(T_, std::vector<char>(block * 21));
+  std::vector<int64_t> len(T_);
   for (int64_t b0 = 0; b0 < nb; b0 += T_) {
     const int64_t b1 = std::min(nb, b0 + T_);
-    std::vector<std::string> out(b1 - b0);
-#pragma omp parallel for schedule(dynamic, 1) num_threads(T_)
+#pragma omp parallel for schedule(static, 1) num_threads(T_)
     for (int64_t b = b0; b < b1; ++b) {
-      std::string& s = out[b - b0];
+      char* s = out[b - b0].data();
+      char* p = s;
       const int64_t lo = b * block, hi = std::min(n, lo + block);
-      s.reserve((hi - lo) * 8);
-      char tmp[24];
-      for (int64_t i = lo; i < hi; ++i) {
-        const int len = std::snprintf(tmp, sizeof tmp, "%lld\n", (long long)a[i]);
-        s.append(tmp, len);
-      }
+      for (int64_t i = lo; i < hi; ++i) p = put_line(p, (long long)a[i]);
+      len[b - b0] = p - s;
     }
-    for (auto& s : out)
-      if (std::fwrite(s.data(), 1, s.size(), f) != s.size()) return fail(PBBS_ERR_IO, "write failed");
+    for (int64_t b = b0; b < b1; ++b)
+      if (std::fwrite(out[b - b0].data(), 1, (size_t)len[b - b0], f) != (size_t)len[b - b0])
+        return fail(PBBS_ERR_IO, "write failed");
   }
   return PBBS_OK;
 }
@@ -131,60 +146,6 @@ int write_lines(FILE* f, const T* a, int64_t n) {
 extern "C" {
 
 void pbbs_free(void* p) { std::free(p); }
-
-int pbbs_read_adjacency_graph(const char* path, pbbs_csr* out) {
-  pbbs::clear_error();
-  if (!path || !out) return fail(PBBS_ERR_INVALID, "NULL argument");
-  std::vector<char> buf;
-  int rc = read_file(path, buf);
-  if (rc != PBBS_OK) return rc;
-  int64_t after = 0;
-  if (header_of(buf, &after) != "AdjacencyGraph")
-    return fail(PBBS_ERR_FORMAT, "%s: missing header AdjacencyGraph", path);
-  const std::vector<int64_t> tok = token_starts(buf, after, (int64_t)buf.size());
-  if (tok.size() < 2) return fail(PBBS_ERR_FORMAT, "%s: missing n and m", path);
-  long long n = 0, m = 0;
-  if (!parse_long(&buf[tok[0]], &n) || !parse_long(&buf[tok[1]], &m) || n < 0 || m < 0)
-    return fail(PBBS_ERR_FORMAT, "%s: bad n or m", path);
-  // the reference requires exactly n + m + 3 tokens (common/graphIO.h:204-206)
-  if ((long long)tok.size() != n + m + 2)
-    return fail(PBBS_ERR_FORMAT, "%s: length = %lld, n+m+3 = %lld", path,
-                (long long)tok.size() + 1, n + m + 3);
-  if (n > INT32_MAX) return fail(PBBS_ERR_LIMIT, "%s: n=%lld >= 2^31", path, n);
-  int64_t* offsets = static_cast<int64_t*>(std::malloc((n + 1) * sizeof(int64_t)));
-  int32_t* adj = static_cast<int32_t*>(std::malloc(std::max<long long>(m, 1) * sizeof(int32_t)));
-  if (!offsets || !adj) {
-    std::free(offsets);
-    std::free(adj);
-    return fail(PBBS_ERR_NOMEM, "host CSR allocation failed");
-  }
-  long long bad = -1;
-#pragma omp parallel for schedule(static) reduction(max : bad)
-  for (long long i = 0; i < n + m; ++i) {
-    long long x = 0;
-    if (!parse_long(&buf[tok[2 + i]], &x)) {
-      bad = std::max(bad, i);
-      continue;
-    }
-    if (i < n)
-      offsets[i] = x;
-    else if (x < INT32_MIN || x > INT32_MAX)
-      bad = std::max(bad, i);
-    else
-      adj[i - n] = (int32_t)x;
-  }
-  if (bad >= 0) {
-    std::free(offsets);
-    std::free(adj);
-    return fail(PBBS_ERR_FORMAT, "%s: token %lld is not a valid integer", path, bad + 3);
-  }
-  offsets[n] = m;  // tagged on, as common/graphIO.h:209-210 does
-  out->n = n;
-  out->m = m;
-  out->offsets = offsets;
-  out->adj = adj;
-  return PBBS_OK;
-}
 
 int pbbs_write_adjacency_graph(const char* path, const pbbs_csr* g) {
   pbbs::clear_error();

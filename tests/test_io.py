@@ -66,3 +66,80 @@ def test_golden_files_parse():
     g = pb.read_graph_from_file(f"{GOLDEN}/torus4.adj")
     assert g.n == 64 and g.m == 384
     assert pb.read_int_seq_from_file(f"{GOLDEN}/torus4.parent").size == 64
+
+
+def test_reader_random_separators_and_edges(tmp_path):
+    """Runs of mixed separators, no trailing newline, tokens falling on the
+    parser's chunk cuts: the parse must equal the tokens written."""
+    rng = np.random.default_rng(7)
+    g = pb.rmat(12, 40000, 0.57, 0.19, 0.19, seed=2)
+    toks = ["AdjacencyGraph", str(g.n), str(g.m)] + [str(x) for x in g.offsets[:-1]] + \
+        [str(x) for x in g.adj]
+    seps = [" ", "\t", "\n", "\r", "\r\n", "  \t"]
+    pick = rng.integers(0, len(seps), len(toks))
+    body = "".join(t + seps[k] for t, k in zip(toks, pick)).rstrip()
+    p = tmp_path / "r.adj"
+    p.write_text("\n \t" + body)
+    h = pb.read_graph_from_file(p)
+    assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.adj, h.adj)
+
+
+@pytest.mark.parametrize("body,n,m", [
+    (b"AdjacencyGraph\n1\n0\n0", 1, 0),
+    (b"AdjacencyGraph 1 0 0", 1, 0),
+    (b"AdjacencyGraph\n2\n2\n0\n1\n1\n0\n", 2, 2),
+])
+def test_reader_small_files(tmp_path, body, n, m):
+    p = tmp_path / "s.adj"
+    p.write_bytes(body)
+    g = pb.read_graph_from_file(p)
+    assert g.n == n and g.m == m and g.offsets[-1] == m
+
+
+@pytest.mark.parametrize("body,match", [
+    (b"AdjacencyGraph\n2\n1\n0\n1\n12abc\n", "integer"),
+    (b"AdjacencyGraph\n2\n1\n0\n1\n1-2\n", "integer"),
+    (b"AdjacencyGraph\n2\n1\n0\n1\n9999999999\n", "integer"),       # beyond int32
+    (b"AdjacencyGraphX\n1\n0\n0\n", "header"),
+    (b"AdjacencyGraph\n", "missing n and m"),
+])
+def test_reader_rejects_bad_tokens(tmp_path, body, match):
+    p = tmp_path / "bad.adj"
+    p.write_bytes(body)
+    with pytest.raises(pb.PbbsError, match=match):
+        pb.read_graph_from_file(p)
+
+
+def test_binary_round_trip(tmp_path):
+    """text -> host CSR -> binary -> host CSR, on a generated graph and on
+    every golden graph."""
+    import glob
+    import os
+    graphs = [pb.rmat(13, 80000, 0.5, 0.1, 0.1, seed=4, permute_seed=9)]
+    graphs += [pb.read_graph_from_file(f) for f in sorted(glob.glob(os.path.join(GOLDEN, "*.adj")))]
+    for i, g in enumerate(graphs):
+        p = tmp_path / f"g{i}.csr"
+        pb.write_graph_binary(g, p)
+        raw = p.read_bytes()
+        assert raw[:8] == b"PBBSCSR1" and len(raw) == 32 + 8 * (g.n + 1) + 4 * g.m
+        assert int.from_bytes(raw[8:16], "little") == g.n
+        assert int.from_bytes(raw[16:24], "little") == g.m
+        assert np.array_equal(np.frombuffer(raw[32:32 + 8 * (g.n + 1)], "<i8"), g.offsets)
+        h = pb.read_graph_binary(p)
+        assert np.array_equal(g.offsets, h.offsets) and np.array_equal(g.adj, h.adj)
+
+
+def test_binary_rejects_bad_files(tmp_path):
+    g = pb.grid3d(3)
+    p = tmp_path / "g.csr"
+    pb.write_graph_binary(g, p)
+    raw = p.read_bytes()
+    (tmp_path / "short.csr").write_bytes(raw[:-4])
+    with pytest.raises(pb.PbbsError, match="bytes"):
+        pb.read_graph_binary(tmp_path / "short.csr")
+    (tmp_path / "magic.csr").write_bytes(b"PBBSCSR2" + raw[8:])
+    with pytest.raises(pb.PbbsError, match="not a PBBSCSR1"):
+        pb.read_graph_binary(tmp_path / "magic.csr")
+    (tmp_path / "text.adj").write_text("AdjacencyGraph\n1\n0\n0\n")
+    with pytest.raises(pb.PbbsError, match="not a PBBSCSR1"):
+        pb.read_graph_binary(tmp_path / "text.adj")
