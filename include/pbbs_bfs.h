@@ -79,7 +79,10 @@ typedef struct {
                                0 = launch each level kernel on the stream */
     int32_t batch_pull_div; /* batched BFS: pull a level when |F| + sum deg(F) > m / this,
                                else push; default 10; 0 = push only */
-    int32_t reserved;
+    int32_t bu_row_tops;    /* 1 = bottom-up levels read a 32-byte per-vertex record (the
+                               six largest neighbours, row end, degree), built in HBM on the
+                               first BFS of a sorted symmetric graph (default); 0 = they read
+                               offsets and row tops from the CSR (no extra HBM) */
 } pbbs_tuning;
 
 int pbbs_tuning_get(const pbbs_ctx* ctx, pbbs_tuning* out);

@@ -33,7 +33,7 @@ class Tuning(C.Structure):
         ("chunk_levels", C.c_int32),
         ("use_graphs", C.c_int32),
         ("batch_pull_div", C.c_int32),
-        ("reserved", C.c_int32),
+        ("bu_row_tops", C.c_int32),
     ]
 
 

@@ -49,6 +49,10 @@ struct BfsState {
 // Everything a level kernel needs.  The init kernel copies it into device
 // memory so one captured HIP graph of level launches serves every run that
 // uses the same workspace.
+// Single-source level kernel variants: unsorted rows (full-row bottom-up
+// scans), sorted rows read from the CSR, sorted rows with row-top records.
+enum LevelVariant { kLevelUnsorted = 0, kLevelSorted = 1, kLevelSortedTops = 2 };
+
 template <typename OffT>
 struct BfsArgs {
   const OffT* off;             // offsets[n+1]
@@ -64,6 +68,8 @@ struct BfsArgs {
   int32_t* host_status;        // pinned host memory: [0] done, [1] levels started
   const uint32_t* bu_bounds;   // first-word index of each wavefront's share in the first
                                // bottom-up level after top-down (num_waves + 1 entries)
+  const uint32_t* bu_tops;     // [4n] row-top records {three largest neighbours or ~0,
+                               // degree}, or null: bottom-up reads offsets and row tops
   int64_t n;
   int64_t m;
   int64_t nwords;              // ceil(n/64)

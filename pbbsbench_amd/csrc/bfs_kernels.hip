@@ -76,14 +76,93 @@ __device__ __forceinline__ unsigned lanes_below(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
-  const unsigned lane = lane_id();
-#pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    const unsigned y = __shfl_up(x, d);
-    if (lane >= (unsigned)d) x += y;
-  }
+// Cross-lane moves through DPP (a VALU operand modifier, a few cycles)
+// instead of ds_bpermute (an LDS round trip).  `old` is what lanes without a
+// source lane (or masked rows) receive.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ unsigned dpp_u32(unsigned x, unsigned old) {
+  return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)x, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int dpp_i32(int x, int old) {
+  return __builtin_amdgcn_update_dpp(old, x, kCtrl, kRowMask, 0xf, false);
+}
+constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
+constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
+
+// One step of a keyed (segmented) inclusive OR scan: lanes take the value
+// of the source lane when it carries the same key.  Keys are non-decreasing
+// over the lanes, so equal keys at both ends mean one segment in between.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ void seg_or_step(unsigned long long& x, int key) {
+  const unsigned lo = dpp_u32<kCtrl, kRowMask>((unsigned)x, 0u);
+  const unsigned hi = dpp_u32<kCtrl, kRowMask>((unsigned)(x >> 32), 0u);
+  const int k2 = dpp_i32<kCtrl, kRowMask>(key, INT32_MIN);
+  if (k2 == key) x |= ((unsigned long long)hi << 32) | lo;
+}
+
+// Inclusive OR of x over the lanes at or below this one with the same key.
+__device__ __forceinline__ unsigned long long seg_or_scan(unsigned long long x, int key) {
+  seg_or_step<kRowShr1, 0xf>(x, key);
+  seg_or_step<kRowShr2, 0xf>(x, key);
+  seg_or_step<kRowShr4, 0xf>(x, key);
+  seg_or_step<kRowShr8, 0xf>(x, key);
+  seg_or_step<kRowBcast15, 0xa>(x, key);
+  seg_or_step<kRowBcast31, 0xc>(x, key);
   return x;
+}
+
+// The value of the lane below (0 for lane 0 or a different key).
+__device__ __forceinline__ unsigned long long seg_prev(unsigned long long x, int key) {
+  const unsigned lo = dpp_u32<kWaveShr1, 0xf>((unsigned)x, 0u);
+  const unsigned hi = dpp_u32<kWaveShr1, 0xf>((unsigned)(x >> 32), 0u);
+  const int k2 = dpp_i32<kWaveShr1, 0xf>(key, INT32_MIN);
+  return k2 == key ? (((unsigned long long)hi << 32) | lo) : 0ull;
+}
+
+// Segmented inclusive OR where each lane knows `dist`, its distance to the
+// first lane of its segment (segments are runs of lanes).  Row-local steps
+// read 0 from lanes outside the 16-lane row (bound_ctrl), the two broadcast
+// steps bring in the previous rows' last lanes.
+template <int kCtrl, int kRowMask, bool kBound>
+__device__ __forceinline__ unsigned dpp_take(unsigned x) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, kBound);
+}
+__device__ __forceinline__ unsigned long long seg_or_scan_d(unsigned long long x, int dist,
+                                                            int lane16, int lane32) {
+  unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
+#define PBBS_SEG_STEP(ctrl, rm, bound, cond)                        \
+  {                                                                 \
+    const unsigned l2 = dpp_take<ctrl, rm, bound>(lo);             \
+    const unsigned h2 = dpp_take<ctrl, rm, bound>(hi);             \
+    if (cond) {                                                     \
+      lo |= l2;                                                     \
+      hi |= h2;                                                     \
+    }                                                               \
+  }
+  PBBS_SEG_STEP(kRowShr1, 0xf, true, dist >= 1)
+  PBBS_SEG_STEP(kRowShr2, 0xf, true, dist >= 2)
+  PBBS_SEG_STEP(kRowShr4, 0xf, true, dist >= 4)
+  PBBS_SEG_STEP(kRowShr8, 0xf, true, dist >= 8)
+  PBBS_SEG_STEP(kRowBcast15, 0xa, false, dist > lane16)
+  PBBS_SEG_STEP(kRowBcast31, 0xc, false, dist > lane32)
+#undef PBBS_SEG_STEP
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Inclusive sum over the wavefront through DPP.
+__device__ __forceinline__ unsigned wave_inclusive_sum_dpp(unsigned x) {
+  x += dpp_u32<kRowShr1, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr2, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr4, 0xf>(x, 0u);
+  x += dpp_u32<kRowShr8, 0xf>(x, 0u);
+  x += dpp_u32<kRowBcast15, 0xa>(x, 0u);
+  x += dpp_u32<kRowBcast31, 0xc>(x, 0u);
+  return x;
+}
+
+__device__ __forceinline__ unsigned wave_inclusive_sum(unsigned x) {
+  return wave_inclusive_sum_dpp(x);
 }
 
 __device__ __forceinline__ int wave_max(int x) {
@@ -110,6 +189,15 @@ __device__ __forceinline__ bool in_set(gptr<const unsigned> bits, int u) {
   return (bits[u >> 5] >> (u & 31)) & 1u;
 }
 
+// Bit u of a bitmap, false for u < 0.  The word is loaded whatever u is (the
+// index is clamped to 0), so a run of these tests issues all its loads before
+// one wait; `u >= 0 && in_set(...)` compiles to one guarded load and one
+// wait per test.
+__device__ __forceinline__ bool in_set_or_none(gptr<const unsigned> bits, int u) {
+  const unsigned w = bits[(u < 0 ? 0 : u) >> 5];
+  return ((w >> (u & 31)) & (unsigned)(u >= 0)) != 0u;   // no short circuit
+}
+
 __device__ __forceinline__ long long global_wave() {
   return (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 }
@@ -120,6 +208,8 @@ __device__ __forceinline__ long long num_waves() {
 // ------------------------------------------------------------- level view
 // The pointers one level uses, resolved once from the rotating buffers so
 // that no array of the argument block is indexed dynamically.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
 template <typename OffT>
 struct LevelView {
   gptr<const OffT> off;
@@ -137,6 +227,7 @@ struct LevelView {
   gptr<uint32_t> d0_next;
   gptr<const unsigned long long> ctr_cur;  // F_k's counters (all sub-queues)
   gptr<const uint32_t> bu_bounds;       // weighted wavefront shares, or null for equal shares
+  gptr<const u32x4> tops;               // per vertex {three largest neighbours, degree}, or null
   long long n;
   long long nwords;
   long long qcap;
@@ -199,7 +290,7 @@ __device__ __forceinline__ void stage_push(const View& a, SB& sb, Stage& s, bool
   if (mask == 0) return;
   const unsigned c = (unsigned)__popcll(mask);
   const unsigned incl = wave_inclusive_sum(take ? deg : 0u);
-  const unsigned total = __shfl(incl, kWave - 1);
+  const unsigned total = (unsigned)__builtin_amdgcn_readlane((int)incl, kWave - 1);
   if (s.cnt + c > (unsigned)SB::kCapacity) stage_flush_wave(a, sb, s);
   if (take) {
     const int w = threadIdx.x / kWave;
@@ -359,12 +450,17 @@ template <int N, typename OffT>
 __device__ __forceinline__ int scan_top(const LevelView<OffT>& a, OffT lo, OffT& p) {
   const gptr<const unsigned> fbc = a.fb_cur;
   const OffT have = p - lo;
+  // unconditional loads (entry 0 stands in past the row's start) so that all
+  // N are in flight together
   int c[N];
 #pragma unroll
-  for (int j = 0; j < N; ++j) c[j] = (OffT)j < have ? a.adj[p - 1 - j] : -1;
+  for (int j = 0; j < N; ++j) {
+    const int x = a.adj[(OffT)j < have ? p - 1 - (OffT)j : (OffT)0];
+    c[j] = (OffT)j < have ? x : -1;
+  }
   bool h[N];
 #pragma unroll
-  for (int j = 0; j < N; ++j) h[j] = c[j] >= 0 && in_set(fbc, c[j]);
+  for (int j = 0; j < N; ++j) h[j] = in_set_or_none(fbc, c[j]);
   int found = -1;
 #pragma unroll
   for (int j = N - 1; j >= 0; --j)
@@ -439,11 +535,24 @@ __device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine
   return found;
 }
 
+// Unreached vertices a lane probes per pass of a sorted bottom-up level.  The
+// pass issues every load of all of them (offsets, then row tops, then bitmap
+// words) before it uses any, so a wavefront keeps kBuPer times as many
+// independent row fetches in flight: the first bottom-up level is bound by
+// the latency of one random row-top line per unreached vertex.
+#ifndef PBBS_BU_PER_LANE
+#define PBBS_BU_PER_LANE 2
+#endif
+constexpr int kBuPer = PBBS_BU_PER_LANE;
+constexpr int kBuCap = (kBuPer + 1) * kWave;   // queue and park capacity per wavefront
+constexpr int kTopEntries = 6;                 // neighbours held by a row-top record
+
 template <typename OffT>
 struct BuLds {
-  int32_t q[kWavesPerBlock][2 * kWave];      // unreached vertices, ascending
-  int32_t lv[kWavesPerBlock][2 * kWave];     // parked vertices
-  OffT lp[kWavesPerBlock][2 * kWave];        // their scan positions
+  int32_t q[kWavesPerBlock][kBuCap];      // unreached vertices, ascending
+  int32_t lv[kWavesPerBlock][kBuCap];     // parked vertices
+  uint32_t llo[kWavesPerBlock][kBuCap];   // their row starts
+  uint32_t lp[kWavesPerBlock][kBuCap];    // their scan positions (entries [lo, p) untested)
 };
 
 // Records the found lanes: parent/depth, their F_{k+1} bits (one atomicOr per
@@ -459,25 +568,23 @@ __device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb
   }
   const unsigned long long fm = __ballot(nw);
   if (fm) {
-    const int wi = valid ? (v >> 6) : -1 - (int)lane;  // idle lanes join no word
+    // OR of the bits over each run of lanes with the same word (DPP scan);
+    // the run's last lane holds the total and sets it.  Idle lanes (all at
+    // the end) take a key of their own.
+    const int key = valid ? (v >> 6) : INT32_MAX;
     unsigned long long bits = nw ? (1ull << (v & 63)) : 0ull;
-    // OR of `bits` over the lanes at or above this one with the same word
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-      const unsigned long long o = __shfl_down(bits, d);
-      const int ow = __shfl_down(wi, d);
-      if (lane + d < (unsigned)kWave && ow == wi) bits |= o;
-    }
-    const int pw = __shfl_up(wi, 1);
-    const bool head = lane == 0 || pw != wi;
-    if (head && bits) atomicOr(gen(&a.fb_next[wi]), bits);
+    bits = seg_or_scan(bits, key);
+    const int kp = dpp_i32<kWaveShr1, 0xf>(key, INT32_MIN);
+    const unsigned long long heads = __ballot(kp != key);
+    const bool tail = lane == (unsigned)(kWave - 1) || ((heads >> (lane + 1)) & 1ull);
+    if (tail && bits) atomicOr(gen(&a.fb_next[key]), bits);
   }
   stage_push(a, sb, stg, nw, v, deg);
 }
 
 template <typename OffT>
 __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
-                                 BuLds<OffT>& L, unsigned& npark) {
+                                 BuLds<OffT>& L, unsigned& npark, int tested) {
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   wave_lds_sync();
@@ -487,13 +594,13 @@ __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& 
     OffT lo = 0, hi = 0, p = 0;
     if (mine) {
       v = L.lv[w][base + lane];
-      p = L.lp[w][base + lane];
-      lo = a.off[v];
-      hi = a.off[v + 1];
+      lo = (OffT)L.llo[w][base + lane];
+      p = (OffT)L.lp[w][base + lane];
+      hi = p + (OffT)tested;
     }
     int found = -1;
     // parked rows are the deep ones: sixteen loads in flight per pass
-    for (int t = 4; t < a.serial_depth; t += 16) {
+    for (int t = tested; t < a.serial_depth; t += 16) {
       const bool active = mine && found < 0 && p > lo;
       if (!__ballot(active)) break;
       if (active) found = scan_top<16>(a, lo, p);
@@ -505,10 +612,136 @@ __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& 
   npark = 0;
 }
 
-// One probe pass over queue entries [0, cnt) (cnt <= 64).
-template <typename OffT, bool kSorted>
-__device__ void bu_batch(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
+// Parks the lanes with `park` set: vertex v with row [lo, hi), its top
+// `tested` entries already tested.  Finishes the parked rows first when they
+// would overflow the list.  Offsets fit 32 bits (m < 2^32).
+template <typename OffT>
+__device__ __forceinline__ void bu_park(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                        BuLds<OffT>& L, unsigned& npark, bool park, int v,
+                                        uint32_t lo, uint32_t hi, int tested) {
+  const unsigned long long pm = __ballot(park);
+  if (!pm) return;
+  const int w = threadIdx.x / kWave;
+  if (npark + (unsigned)__popcll(pm) > (unsigned)kBuCap)
+    bu_finish_parked(a, sb, stg, L, npark, tested);
+  if (park) {
+    const unsigned slot = npark + lanes_below(pm);
+    L.lv[w][slot] = v;
+    L.llo[w][slot] = lo;
+    L.lp[w][slot] = hi - (uint32_t)tested;
+  }
+  npark += (unsigned)__popcll(pm);
+}
+
+// One probe pass of a sorted level over queue entries [0, cnt), cnt <=
+// kBuPer * 64: lane l takes entries l, l + 64, ...  Each row's top four
+// entries are tested; a row with no hit there and entries left is parked.
+template <typename OffT>
+__device__ void bu_probe(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
                          unsigned cnt, unsigned& npark) {
+  constexpr int N = 4;
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const gptr<const unsigned> fbc = a.fb_cur;
+  wave_lds_sync();
+  bool mine[kBuPer];
+  int v[kBuPer];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    mine[b] = lane + (unsigned)(b * kWave) < cnt;
+    v[b] = mine[b] ? L.q[w][lane + b * kWave] : 0;
+  }
+  OffT lo[kBuPer], hi[kBuPer];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    lo[b] = 0;
+    hi[b] = 0;
+    if (mine[b]) {
+      lo[b] = a.off[v[b]];
+      hi[b] = a.off[v[b] + 1];
+    }
+  }
+  int c[kBuPer][N];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b)
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+      const bool in_row = (OffT)j < hi[b] - lo[b];
+      const int x = a.adj[in_row ? hi[b] - 1 - (OffT)j : (OffT)0];
+      c[b][j] = in_row ? x : -1;
+    }
+  bool h[kBuPer][N];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b)
+#pragma unroll
+    for (int j = 0; j < N; ++j) h[b][j] = in_set_or_none(fbc, c[b][j]);
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    int found = -1;
+#pragma unroll
+    for (int j = N - 1; j >= 0; --j)
+      if (h[b][j]) found = c[b][j];
+    const bool park = mine[b] && found < 0 && hi[b] - lo[b] > (OffT)N;
+    bu_commit(a, sb, stg, mine[b], mine[b] && found >= 0, v[b], found, (uint32_t)(hi[b] - lo[b]));
+    bu_park(a, sb, stg, L, npark, park, v[b], (uint32_t)lo[b], (uint32_t)hi[b], N);
+  }
+}
+
+// Probe pass that reads each vertex's 32-byte row-top record {six largest
+// neighbours (~0 where the row is shorter), row end, degree} in place of its
+// offsets and row: two coalesced loads and six bitmap tests per vertex.  A
+// row longer than six with no hit is parked with its position from the
+// record, so no offsets are read at all.
+template <typename OffT>
+__device__ void bu_probe_tops(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
+                              unsigned cnt, unsigned& npark) {
+  constexpr int N = kTopEntries;
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const gptr<const unsigned> fbc = a.fb_cur;
+  wave_lds_sync();
+  bool mine[kBuPer];
+  int v[kBuPer];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    mine[b] = lane + (unsigned)(b * kWave) < cnt;
+    v[b] = mine[b] ? L.q[w][lane + b * kWave] : 0;
+  }
+  u32x4 r0[kBuPer], r1[kBuPer];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    r0[b] = (u32x4){~0u, ~0u, ~0u, ~0u};
+    r1[b] = (u32x4){~0u, ~0u, 0u, 0u};
+    if (mine[b]) {
+      r0[b] = a.tops[2 * (long long)v[b]];
+      r1[b] = a.tops[2 * (long long)v[b] + 1];
+    }
+  }
+  bool h[kBuPer][N];
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    const unsigned t[N] = {r0[b].x, r0[b].y, r0[b].z, r0[b].w, r1[b].x, r1[b].y};
+#pragma unroll
+    for (int j = 0; j < N; ++j) h[b][j] = in_set_or_none(fbc, (int)t[j]);
+  }
+#pragma unroll
+  for (int b = 0; b < kBuPer; ++b) {
+    const unsigned t[N] = {r0[b].x, r0[b].y, r0[b].z, r0[b].w, r1[b].x, r1[b].y};
+    int found = -1;
+#pragma unroll
+    for (int j = N - 1; j >= 0; --j)
+      if (h[b][j]) found = (int)t[j];
+    const uint32_t hi = r1[b].z, deg = r1[b].w;
+    const bool park = mine[b] && found < 0 && deg > (uint32_t)N;
+    bu_commit(a, sb, stg, mine[b], mine[b] && found >= 0, v[b], found, deg);
+    bu_park(a, sb, stg, L, npark, park, v[b], hi - deg, hi, N);
+  }
+}
+
+// One pass of an unsorted level over queue entries [0, cnt), cnt <= 64.
+template <typename OffT>
+__device__ void bu_batch_unsorted(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                  BuLds<OffT>& L, unsigned cnt) {
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   wave_lds_sync();
@@ -520,30 +753,25 @@ __device__ void bu_batch(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuL
     lo = a.off[v];
     hi = a.off[v + 1];
   }
-  int found = -1;
-  if (kSorted) {
-    OffT p = hi;
-    if (mine && p > lo) found = scan_top<4>(a, lo, p);
-    const bool park = mine && found < 0 && p > lo;
-    const unsigned long long pm = __ballot(park);
-    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
-    if (pm) {
-      if (npark + (unsigned)__popcll(pm) > 2 * kWave) bu_finish_parked(a, sb, stg, L, npark);
-      if (park) {
-        const unsigned slot = npark + lanes_below(pm);
-        L.lv[w][slot] = v;
-        L.lp[w][slot] = p;
-      }
-      npark += (unsigned)__popcll(pm);
-    }
-  } else {
-    found = scan_unsorted(a, mine, lo, hi);
-    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
-  }
+  const int found = scan_unsorted(a, mine, lo, hi);
+  bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
 }
 
-template <typename OffT, bool kSorted>
+template <typename OffT, bool kSorted, bool kTops>
+__device__ __forceinline__ void bu_pass(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                        BuLds<OffT>& L, unsigned cnt, unsigned& npark) {
+  if constexpr (!kSorted)
+    bu_batch_unsorted(a, sb, stg, L, cnt);
+  else if constexpr (kTops)
+    bu_probe_tops(a, sb, stg, L, cnt, npark);
+  else
+    bu_probe(a, sb, stg, L, cnt, npark);
+}
+
+template <typename OffT, bool kSorted, bool kTops>
 __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L) {
+  // entries per pass: kBuPer per lane on sorted rows, one on unsorted rows
+  constexpr unsigned kPass = (unsigned)((kSorted ? kBuPer : 1) * kWave);
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   const gptr<const unsigned long long> fbc64 = (gptr<const unsigned long long>)a.fb_cur;
@@ -568,32 +796,23 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
       const unsigned long long word = __shfl(unv, j);
       const bool take = (word >> lane) & 1ull;
       const unsigned c = (unsigned)__popcll(word);
-      if (nq + c > 2 * kWave) {
-        // keep the queue below two batches: run the full batch first
-        bu_batch<OffT, kSorted>(a, sb, stg, L, kWave, npark);
-        wave_lds_sync();
-        const unsigned rest = nq - kWave;
-        int tmp = lane < rest ? L.q[w][kWave + lane] : 0;
-        wave_lds_sync();
-        if (lane < rest) L.q[w][lane] = tmp;
-        nq = rest;
-      }
+      // nq < kPass here, so the queue (kPass + 64 entries) has room for c <= 64
       if (take) L.q[w][nq + lanes_below(word)] = (int)((g0 + j) * kWave + lane);
       nq += c;
-      if (nq >= (unsigned)kWave) {
-        bu_batch<OffT, kSorted>(a, sb, stg, L, kWave, npark);
+      if (nq >= kPass) {
+        bu_pass<OffT, kSorted, kTops>(a, sb, stg, L, kPass, npark);
         wave_lds_sync();
-        const unsigned rest = nq - kWave;
-        int tmp = lane < rest ? L.q[w][kWave + lane] : 0;
+        const unsigned rest = nq - kPass;   // < 64
+        int tmp = lane < rest ? L.q[w][kPass + lane] : 0;
         wave_lds_sync();
         if (lane < rest) L.q[w][lane] = tmp;
         nq = rest;
       }
     }
   }
-  if (nq) bu_batch<OffT, kSorted>(a, sb, stg, L, nq, npark);
+  if (nq) bu_pass<OffT, kSorted, kTops>(a, sb, stg, L, nq, npark);
   PBBS_STAMP(a.k, 1);
-  if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark);
+  if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark, kTops ? kTopEntries : 4);
 }
 
 // -------------------------------------------------------------- level driver
@@ -609,7 +828,7 @@ __device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsi
 #ifndef PBBS_LEVEL_MIN_BLOCKS
 #define PBBS_LEVEL_MIN_BLOCKS 1
 #endif
-template <typename OffT, bool kSorted>
+template <typename OffT, bool kSorted, bool kTops>
 __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
@@ -696,6 +915,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   // cut at equal sums of 1 + log2(1 + degree).  Later bottom-up levels meet
   // few, low-degree vertices and split the words evenly.
   a.bu_bounds = glob((prev == kTopDown) ? A->bu_bounds : nullptr);
+  a.tops = glob(kTops ? reinterpret_cast<const u32x4*>(A->bu_tops) : nullptr);
   a.n = A->n;
   a.nwords = A->nwords;
   a.k = k;
@@ -704,7 +924,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   if (mode == kTopDown)
     top_down(a, sb, stg, E);
   else
-    bottom_up<OffT, kSorted>(a, sb, stg, bl);
+    bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
   PBBS_STAMP(k, 2);
   stage_flush_block(a, sb, stg);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
@@ -887,7 +1107,6 @@ __device__ __forceinline__ void put_depth(const MbfsView<OffT>& a, int i, long l
   }
 }
 
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // 0xFF in each byte of a 32-bit word whose bit is set in the low 4 bits of b.
 __device__ __forceinline__ unsigned byte_mask4(unsigned b) {
@@ -898,91 +1117,6 @@ __device__ __forceinline__ unsigned byte_mask4(unsigned b) {
 __device__ __forceinline__ unsigned long long wave_or64(unsigned long long x) {
 #pragma unroll
   for (int d = 1; d < kWave; d <<= 1) x |= __shfl_xor(x, d);
-  return x;
-}
-
-// Cross-lane moves through DPP (a VALU operand modifier, a few cycles)
-// instead of ds_bpermute (an LDS round trip).  `old` is what lanes without a
-// source lane (or masked rows) receive.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ unsigned dpp_u32(unsigned x, unsigned old) {
-  return (unsigned)__builtin_amdgcn_update_dpp((int)old, (int)x, kCtrl, kRowMask, 0xf, false);
-}
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ int dpp_i32(int x, int old) {
-  return __builtin_amdgcn_update_dpp(old, x, kCtrl, kRowMask, 0xf, false);
-}
-constexpr int kRowShr1 = 0x111, kRowShr2 = 0x112, kRowShr4 = 0x114, kRowShr8 = 0x118;
-constexpr int kRowBcast15 = 0x142, kRowBcast31 = 0x143, kWaveShr1 = 0x138;
-
-// One step of a keyed (segmented) inclusive OR scan: lanes take the value
-// of the source lane when it carries the same key.  Keys are non-decreasing
-// over the lanes, so equal keys at both ends mean one segment in between.
-template <int kCtrl, int kRowMask>
-__device__ __forceinline__ void seg_or_step(unsigned long long& x, int key) {
-  const unsigned lo = dpp_u32<kCtrl, kRowMask>((unsigned)x, 0u);
-  const unsigned hi = dpp_u32<kCtrl, kRowMask>((unsigned)(x >> 32), 0u);
-  const int k2 = dpp_i32<kCtrl, kRowMask>(key, INT32_MIN);
-  if (k2 == key) x |= ((unsigned long long)hi << 32) | lo;
-}
-
-// Inclusive OR of x over the lanes at or below this one with the same key.
-__device__ __forceinline__ unsigned long long seg_or_scan(unsigned long long x, int key) {
-  seg_or_step<kRowShr1, 0xf>(x, key);
-  seg_or_step<kRowShr2, 0xf>(x, key);
-  seg_or_step<kRowShr4, 0xf>(x, key);
-  seg_or_step<kRowShr8, 0xf>(x, key);
-  seg_or_step<kRowBcast15, 0xa>(x, key);
-  seg_or_step<kRowBcast31, 0xc>(x, key);
-  return x;
-}
-
-// The value of the lane below (0 for lane 0 or a different key).
-__device__ __forceinline__ unsigned long long seg_prev(unsigned long long x, int key) {
-  const unsigned lo = dpp_u32<kWaveShr1, 0xf>((unsigned)x, 0u);
-  const unsigned hi = dpp_u32<kWaveShr1, 0xf>((unsigned)(x >> 32), 0u);
-  const int k2 = dpp_i32<kWaveShr1, 0xf>(key, INT32_MIN);
-  return k2 == key ? (((unsigned long long)hi << 32) | lo) : 0ull;
-}
-
-// Segmented inclusive OR where each lane knows `dist`, its distance to the
-// first lane of its segment (segments are runs of lanes).  Row-local steps
-// read 0 from lanes outside the 16-lane row (bound_ctrl), the two broadcast
-// steps bring in the previous rows' last lanes.
-template <int kCtrl, int kRowMask, bool kBound>
-__device__ __forceinline__ unsigned dpp_take(unsigned x) {
-  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, kCtrl, kRowMask, 0xf, kBound);
-}
-__device__ __forceinline__ unsigned long long seg_or_scan_d(unsigned long long x, int dist,
-                                                            int lane16, int lane32) {
-  unsigned lo = (unsigned)x, hi = (unsigned)(x >> 32);
-#define PBBS_SEG_STEP(ctrl, rm, bound, cond)                        \
-  {                                                                 \
-    const unsigned l2 = dpp_take<ctrl, rm, bound>(lo);             \
-    const unsigned h2 = dpp_take<ctrl, rm, bound>(hi);             \
-    if (cond) {                                                     \
-      lo |= l2;                                                     \
-      hi |= h2;                                                     \
-    }                                                               \
-  }
-  PBBS_SEG_STEP(kRowShr1, 0xf, true, dist >= 1)
-  PBBS_SEG_STEP(kRowShr2, 0xf, true, dist >= 2)
-  PBBS_SEG_STEP(kRowShr4, 0xf, true, dist >= 4)
-  PBBS_SEG_STEP(kRowShr8, 0xf, true, dist >= 8)
-  PBBS_SEG_STEP(kRowBcast15, 0xa, false, dist > lane16)
-  PBBS_SEG_STEP(kRowBcast31, 0xc, false, dist > lane32)
-#undef PBBS_SEG_STEP
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-// Inclusive sum over the wavefront through DPP.
-__device__ __forceinline__ unsigned wave_inclusive_sum_dpp(unsigned x) {
-  x += dpp_u32<kRowShr1, 0xf>(x, 0u);
-  x += dpp_u32<kRowShr2, 0xf>(x, 0u);
-  x += dpp_u32<kRowShr4, 0xf>(x, 0u);
-  x += dpp_u32<kRowShr8, 0xf>(x, 0u);
-  x += dpp_u32<kRowBcast15, 0xa>(x, 0u);
-  x += dpp_u32<kRowBcast31, 0xc>(x, 0u);
   return x;
 }
 
@@ -1855,6 +1989,26 @@ __global__ __launch_bounds__(kBlock) void word_weight_kernel(const OffT* __restr
   }
 }
 
+// Row-top records for sorted rows, two 16-byte halves per vertex:
+// out[2v] = {adj[hi-1], adj[hi-2], adj[hi-3], adj[hi-4]}, out[2v+1] =
+// {adj[hi-5], adj[hi-6], hi, degree}, hi = off[v+1], ~0 where the row is
+// shorter.  hi fits 32 bits (m < 2^32).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void row_tops_kernel(const OffT* __restrict__ off,
+                                                          const int32_t* __restrict__ adj,
+                                                          long long n, u32x4* __restrict__ out) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    const OffT lo = off[v], hi = off[v + 1];
+    const OffT d = hi - lo;
+    unsigned t[kTopEntries];
+#pragma unroll
+    for (int j = 0; j < kTopEntries; ++j) t[j] = d > (OffT)j ? (unsigned)adj[hi - 1 - j] : ~0u;
+    out[2 * v] = (u32x4){t[0], t[1], t[2], t[3]};
+    out[2 * v + 1] = (u32x4){t[4], t[5], (unsigned)hi, (unsigned)d};
+  }
+}
+
 // bounds[i] = first word j with prefix[j] >= total * i / waves (prefix has
 // nwords + 1 entries, exclusive).
 __global__ __launch_bounds__(kBlock) void share_bounds_kernel(const unsigned long long* __restrict__ prefix,
@@ -1881,17 +2035,22 @@ hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
 }
 
 template <typename OffT>
-const void* level_kernel_symbol(bool sorted) {
-  return sorted ? reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true>)
-                : reinterpret_cast<const void*>(&bfs_level_kernel<OffT, false>);
+const void* level_kernel_symbol(int variant) {
+  switch (variant) {
+    case kLevelSortedTops: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true, true>);
+    case kLevelSorted: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true, false>);
+    default: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, false, false>);
+  }
 }
 
 template <typename OffT>
-hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, bool sorted, int grid, hipStream_t stream) {
-  if (sorted)
-    hipLaunchKernelGGL((bfs_level_kernel<OffT, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, int variant, int grid, hipStream_t stream) {
+  if (variant == kLevelSortedTops)
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, true, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+  else if (variant == kLevelSorted)
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, true, false>), dim3(grid), dim3(kBlock), 0, stream, dargs);
   else
-    hipLaunchKernelGGL((bfs_level_kernel<OffT, false>), dim3(grid), dim3(kBlock), 0, stream, dargs);
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, false, false>), dim3(grid), dim3(kBlock), 0, stream, dargs);
   return hipGetLastError();
 }
 
@@ -1956,6 +2115,14 @@ hipError_t launch_share_bounds(const unsigned long long* prefix, long long nword
   const long long threads = waves + 1;
   hipLaunchKernelGGL(share_bounds_kernel, dim3((unsigned)((threads + kBlock - 1) / kBlock)),
                      dim3(kBlock), 0, stream, prefix, nwords, waves, bounds);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+hipError_t launch_row_tops(const OffT* off, const int32_t* adj, long long n, uint32_t* out,
+                           int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(row_tops_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, off, adj, n,
+                     reinterpret_cast<u32x4*>(out));
   return hipGetLastError();
 }
 
@@ -2087,8 +2254,8 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
   template hipError_t launch_bfs_init<OffT>(const BfsArgs<OffT>&, BfsArgs<OffT>*,                \
                                             const unsigned long long*, int32_t, int,             \
                                             hipStream_t);                                        \
-  template const void* level_kernel_symbol<OffT>(bool);                                         \
-  template hipError_t launch_bfs_level<OffT>(const BfsArgs<OffT>*, bool, int, hipStream_t);      \
+  template const void* level_kernel_symbol<OffT>(int);                                          \
+  template hipError_t launch_bfs_level<OffT>(const BfsArgs<OffT>*, int, int, hipStream_t);       \
   template hipError_t launch_graph_offsets<OffT>(const OffT*, long long, long long,             \
                                                  unsigned long long*, unsigned long long*, int,  \
                                                  hipStream_t);                                   \
@@ -2109,7 +2276,9 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
                                                hipStream_t);                                    \
   template hipError_t launch_graph_symmetry<OffT>(const OffT*, const int32_t*, long long,        \
                                                   long long, unsigned long long*, int,           \
-                                                  hipStream_t);
+                                                  hipStream_t);                                  \
+  template hipError_t launch_row_tops<OffT>(const OffT*, const int32_t*, long long, uint32_t*,   \
+                                            int, hipStream_t);
 
 PBBS_INSTANTIATE(uint32_t)
 PBBS_INSTANTIATE(int64_t)

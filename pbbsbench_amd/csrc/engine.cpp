@@ -24,9 +24,9 @@ template <typename OffT>
 hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
                            const unsigned long long* iso, int32_t s, int grid, hipStream_t stream);
 template <typename OffT>
-const void* level_kernel_symbol(bool sorted);
+const void* level_kernel_symbol(int variant);
 template <typename OffT>
-hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, bool sorted, int grid, hipStream_t stream);
+hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, int variant, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_graph_offsets(const OffT* off, long long n, long long m,
                                 unsigned long long* iso, unsigned long long* counters, int grid,
@@ -60,6 +60,9 @@ hipError_t exclusive_scan_u64(const unsigned long long* in, unsigned long long* 
                               hipStream_t stream);
 hipError_t launch_share_bounds(const unsigned long long* prefix, long long nwords, long long waves,
                                uint32_t* bounds, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_row_tops(const OffT* off, const int32_t* adj, long long n, uint32_t* out,
+                           int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long n, long long m,
                                  unsigned long long* counters, int grid, hipStream_t stream);
@@ -116,7 +119,7 @@ struct pbbs_ctx {
   struct Run {
     int kind = 0;  // 0 single-source, 1 batched
     bool off64 = false;
-    bool sorted = false;
+    int sorted = 0;   // single source: LevelVariant; batched: rows sorted
     int grid = 0;
     int K = 0;
     int64_t launched = 0;
@@ -153,6 +156,7 @@ struct pbbs_graph {
   std::mutex meta_mu;
   std::map<int64_t, uint32_t*> bounds;   // weighted bottom-up shares (single source)
   std::map<int64_t, int32_t*> pull_rows; // first row of each edge slice (batched pull)
+  uint32_t* d_tops = nullptr;            // single source: row-top records, 32 B per vertex
   int32_t* d_rank = nullptr;             // batched BFS: decreasing-degree rank of each vertex
   int32_t* d_inv_rank = nullptr;         // batched BFS: vertex of each rank
   int32_t* d_adj_rank = nullptr;         // batched BFS: adj in rank ids
@@ -201,6 +205,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->chunk_levels = 16;
   t->use_graphs = 1;
   t->batch_pull_div = 10;
+  t->bu_row_tops = 1;
   return PBBS_OK;
 }
 
@@ -280,10 +285,12 @@ int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
   if (!c || !t) return fail(PBBS_ERR_INVALID, "NULL argument");
   if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
       t->bu_serial_depth < 0 || t->bu_serial_depth > 4096 || t->chunk_levels < 1 ||
-      t->chunk_levels > 4096 || t->batch_pull_div < 0)
+      t->chunk_levels > 4096 || t->batch_pull_div < 0 || t->bu_row_tops < 0 ||
+      t->bu_row_tops > 1)
     return fail(PBBS_ERR_INVALID,
                 "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
-                "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096, batch_pull_div>=0)");
+                "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096, batch_pull_div>=0, "
+                "bu_row_tops 0 or 1)");
   c->tuning = *t;
   return PBBS_OK;
 }
@@ -380,6 +387,7 @@ int pbbs_graph_free(pbbs_graph* g) {
   (void)hipFree(g->d_wprefix);
   for (auto& kv : g->bounds) (void)hipFree(kv.second);
   for (auto& kv : g->pull_rows) (void)hipFree(kv.second);
+  (void)hipFree(g->d_tops);
   (void)hipFree(g->d_rank);
   (void)hipFree(g->d_inv_rank);
   (void)hipFree(g->d_adj_rank);
@@ -448,10 +456,11 @@ int pbbs_graph_info_get(const pbbs_graph* g, pbbs_graph_info* out) {
 }  // extern "C"
 
 // ---------------------------------------------------------------------- BFS
-static int level_grid(pbbs_ctx* c, bool off64, bool sorted) {
+static int level_grid(pbbs_ctx* c, bool off64, int variant) {
   if (c->tuning.grid_blocks > 0) return std::min(c->tuning.grid_blocks, kMaxGrid);
   int per_cu = 0;
-  const void* fn = off64 ? level_kernel_symbol<int64_t>(sorted) : level_kernel_symbol<uint32_t>(sorted);
+  const void* fn =
+      off64 ? level_kernel_symbol<int64_t>(variant) : level_kernel_symbol<uint32_t>(variant);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -459,8 +468,8 @@ static int level_grid(pbbs_ctx* c, bool off64, bool sorted) {
 }
 
 template <typename OffT>
-static int get_level_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphExec_t* out) {
-  const auto key = std::make_tuple((int)sizeof(OffT), (int)sorted, grid, K);
+static int get_level_graph(pbbs_ctx* c, int variant, int grid, int K, hipGraphExec_t* out) {
+  const auto key = std::make_tuple((int)sizeof(OffT), variant, grid, K);
   auto it = c->level_graphs.find(key);
   if (it != c->level_graphs.end()) {
     *out = it->second;
@@ -470,7 +479,7 @@ static int get_level_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphEx
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   hipError_t le = hipSuccess;
   for (int i = 0; i < K && le == hipSuccess; ++i)
-    le = launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), sorted, grid,
+    le = launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), variant, grid,
                                 c->stream);
   const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
   if (le != hipSuccess || ce != hipSuccess) {
@@ -512,6 +521,38 @@ static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32
   }
   g->bounds[waves] = d;
   *out = d;
+  return PBBS_OK;
+}
+
+// Builds the row-top records of a sorted, symmetric graph on first use
+// (bottom-up levels read them in place of offsets and row tops); null when
+// the graph never runs bottom-up or the tuning turns them off.
+template <typename OffT>
+static int ensure_tops(pbbs_ctx* c, pbbs_graph* g, const uint32_t** out) {
+  *out = nullptr;
+  if (!g->sorted || !g->symmetric || c->tuning.bu_row_tops == 0 || c->tuning.td_to_bu_div <= 0)
+    return PBBS_OK;
+  std::lock_guard<std::mutex> lk(g->meta_mu);
+  if (!g->d_tops) {
+    uint32_t* d = nullptr;
+    const size_t bytes = (size_t)g->n * 32;
+    if (hipMalloc(&d, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PBBS_ERR_NOMEM, "row-top records (%zu bytes)", bytes);
+    }
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock,
+                                                                  c->num_cus * 8));
+    hipError_t e = launch_row_tops<OffT>(static_cast<const OffT*>(g->d_off), g->d_adj, g->n, d,
+                                         grid, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(d);
+      return fail(PBBS_ERR_HIP, "row-top records: %s", hipGetErrorString(e));
+    }
+    g->d_tops = d;
+    g->device_bytes += (int64_t)bytes;
+  }
+  *out = g->d_tops;
   return PBBS_OK;
 }
 
@@ -577,7 +618,7 @@ static int ensure_ranks(pbbs_ctx* c, pbbs_graph* g) {
 
 template <typename OffT>
 static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth,
-                               const uint32_t* bounds) {
+                               const uint32_t* bounds, const uint32_t* tops) {
   const int64_t nwords = (g->n + kWave - 1) / kWave;
   BfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
@@ -598,6 +639,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
   a.bu_bounds = bounds;
+  a.bu_tops = tops;
   a.n = g->n;
   a.m = g->m;
   a.nwords = nwords;
@@ -691,19 +733,27 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
   return launch_chunk(c);
 }
 
+static int level_variant(const pbbs_graph* g, const uint32_t* tops) {
+  return !g->sorted ? kLevelUnsorted : tops ? kLevelSortedTops : kLevelSorted;
+}
+
 template <typename OffT>
 static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth) {
-  const int grid = level_grid(c, g->off64, g->sorted);
-  const uint32_t* bounds = nullptr;
-  int rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+  const uint32_t* tops = nullptr;
+  int rc = ensure_tops<OffT>(c, g, &tops);
   if (rc != PBBS_OK) return rc;
-  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth, bounds);
+  const int variant = level_variant(g, tops);
+  const int grid = level_grid(c, g->off64, variant);
+  const uint32_t* bounds = nullptr;
+  rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+  if (rc != PBBS_OK) return rc;
+  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth, bounds, tops);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   pbbs_ctx::Run& r = c->run;
   r.kind = 0;
   r.off64 = g->off64;
-  r.sorted = g->sorted;
+  r.sorted = variant;
   r.grid = grid;
   r.K = c->tuning.chunk_levels;
   r.launched = 0;
@@ -711,7 +761,7 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.limit = g->n + 2;
   if (c->tuning.use_graphs) {  // capture (first use only) while the stream is idle
     hipGraphExec_t exec = nullptr;
-    rc = get_level_graph<OffT>(c, g->sorted, grid, r.K, &exec);
+    rc = get_level_graph<OffT>(c, variant, grid, r.K, &exec);
     if (rc != PBBS_OK) return rc;
   }
   volatile int32_t* hs = c->host_status;
@@ -787,11 +837,15 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
 template <typename OffT>
 static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
                        int32_t max_levels, int32_t* count, double* init_ms) {
-  const int grid = level_grid(c, g->off64, g->sorted);
-  const uint32_t* bounds = nullptr;
-  int brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+  const uint32_t* tops = nullptr;
+  int brc = ensure_tops<OffT>(c, g, &tops);
   if (brc != PBBS_OK) return brc;
-  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth, bounds);
+  const int variant = level_variant(g, tops);
+  const int grid = level_grid(c, g->off64, variant);
+  const uint32_t* bounds = nullptr;
+  brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+  if (brc != PBBS_OK) return brc;
+  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth, bounds, tops);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   volatile int32_t* hs = c->host_status;
@@ -825,7 +879,7 @@ static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile
     hipEvent_t prev = e1;
     for (int64_t k = 0; k <= g->n + 1; ++k) {
       hipEvent_t e = new_event();
-      if (!e || launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), g->sorted,
+      if (!e || launch_bfs_level<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), variant,
                                        grid, c->stream) != hipSuccess ||
           hipEventRecord(e, c->stream) != hipSuccess || hipEventSynchronize(e) != hipSuccess) {
         rc = fail(PBBS_ERR_HIP, "level launch failed");

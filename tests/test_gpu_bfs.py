@@ -96,6 +96,9 @@ def test_permuted_rmat(ctx):
     dict(grid_blocks=7),
     dict(use_graphs=0),
     dict(chunk_levels=1),
+    dict(bu_row_tops=0),                        # bottom-up reads row tops from the CSR
+    dict(bu_row_tops=0, td_to_bu_div=1_000_000, bu_to_td_div=1_000_000),
+    dict(bu_row_tops=0, bu_serial_depth=0, grid_blocks=3),
 ])
 def test_tuning_invariance(tune):
     say("tuning", tune)
