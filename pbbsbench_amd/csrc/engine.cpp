@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -125,6 +126,7 @@ struct pbbs_ctx {
     int64_t launched = 0;
     int64_t limit = 0;
     bool finalized = false;  // batched: the depth pass after the last level is queued
+    pbbs_graph* graph = nullptr;  // learns the launch count when the run is done
   } run;
   // batched BFS
   int64_t cap_bn = 0;                    // vertices the mask buffers hold
@@ -165,6 +167,9 @@ struct pbbs_graph {
   int64_t isolated = 0;
   int64_t max_degree = 0;
   int64_t device_bytes = 0;
+  // Level launches the last single-source [0] / batched [1] run needed (its
+  // levels plus the launch that found the frontier empty); 0 = none yet.
+  std::atomic<int> last_launches[2] = {0, 0};
 };
 
 static void free_workspace(pbbs_ctx* c) {
@@ -722,6 +727,7 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
       if (!wait) return PBBS_OK;
       HIP_TRY(hipEventSynchronize(c->ev_end));
     }
+    if (r.graph) r.graph->last_launches[r.kind] = c->host_status[1] + 1;
     *done = true;
     return PBBS_OK;
   }
@@ -731,6 +737,15 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
                 (long long)r.launched);
   r.K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(r.launched, 1 << 20));
   return launch_chunk(c);
+}
+
+// Launches in a run's first chunk: BFS depths vary little between sources of
+// one graph, so after the first run the chunk covers the last run's launches
+// plus one (a launch past the end costs a few microseconds; one short costs
+// a host round trip), capped by the tuning's chunk_levels.
+static int first_chunk(const pbbs_ctx* c, const pbbs_graph* g, int kind) {
+  const int last = g->last_launches[kind];
+  return last > 0 ? std::min(last + 1, c->tuning.chunk_levels) : c->tuning.chunk_levels;
 }
 
 static int level_variant(const pbbs_graph* g, const uint32_t* tops) {
@@ -755,7 +770,8 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.off64 = g->off64;
   r.sorted = variant;
   r.grid = grid;
-  r.K = c->tuning.chunk_levels;
+  r.K = first_chunk(c, g, r.kind);
+  r.graph = g;
   r.launched = 0;
   r.finalized = false;
   r.limit = g->n + 2;
@@ -1050,7 +1066,8 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
     a.inv_rank = g->d_inv_rank;
     a.adj_rank = g->d_adj_rank;
   }
-  r.K = c->tuning.chunk_levels;
+  r.K = first_chunk(c, g, r.kind);
+  r.graph = g;
   r.launched = 0;
   r.finalized = false;
   r.limit = n + 2;
