@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <tuple>
@@ -102,6 +103,10 @@ struct pbbs_ctx {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   pbbs_tuning tuning{};
+  // graphs uploaded through this context: destroying the context detaches
+  // them (they stay usable from other contexts on the device, and freeable)
+  std::mutex graphs_mu;
+  std::set<pbbs_graph*> graphs;
   // workspace, sized for the largest graph seen so far
   int64_t cap_n = 0;
   int32_t* w_parent = nullptr;
@@ -144,7 +149,8 @@ struct pbbs_ctx {
 };
 
 struct pbbs_graph {
-  pbbs_ctx* ctx = nullptr;
+  pbbs_ctx* ctx = nullptr;   // uploading context, null once it is destroyed
+  int device = 0;
   int64_t n = 0, m = 0;
   bool off64 = false;
   bool sorted = false;
@@ -254,6 +260,11 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   if (!c) return PBBS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  {
+    std::lock_guard<std::mutex> lk(c->graphs_mu);
+    for (pbbs_graph* g : c->graphs) g->ctx = nullptr;
+    c->graphs.clear();
+  }
   for (auto& kv : c->level_graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second);
   free_workspace(c);
@@ -382,9 +393,11 @@ extern "C" {
 
 int pbbs_graph_free(pbbs_graph* g) {
   if (!g) return PBBS_OK;
+  (void)hipSetDevice(g->device);
   if (g->ctx) {
-    (void)hipSetDevice(g->ctx->device);
     (void)hipStreamSynchronize(g->ctx->stream);
+    std::lock_guard<std::mutex> lk(g->ctx->graphs_mu);
+    g->ctx->graphs.erase(g);
   }
   (void)hipFree(g->d_off);
   (void)hipFree(g->d_adj);
@@ -413,6 +426,11 @@ int pbbs_graph_upload(pbbs_ctx* c, int64_t n, int64_t m, const void* offsets, co
   HIP_TRY(hipSetDevice(c->device));
   pbbs_graph* g = new pbbs_graph();
   g->ctx = c;
+  g->device = c->device;
+  {
+    std::lock_guard<std::mutex> lk(c->graphs_mu);
+    c->graphs.insert(g);
+  }
   g->n = n;
   g->m = m;
   g->off64 = (flags & PBBS_GRAPH_OFFSETS_I64) != 0;
@@ -820,7 +838,7 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
                         int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats) {
   clear_error();
   if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
-  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (source < 0 || (int64_t)source >= g->n)
     return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
   HIP_TRY(hipSetDevice(c->device));
@@ -937,7 +955,7 @@ extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
   clear_error();
   if (!c || !g || !count || (max_levels > 0 && !out))
     return fail(PBBS_ERR_INVALID, "NULL argument");
-  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (source < 0 || (int64_t)source >= g->n)
     return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", source, (long long)g->n);
   HIP_TRY(hipSetDevice(c->device));
@@ -1127,7 +1145,7 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
                               pbbs_bfs_stats* stats) {
   clear_error();
   if (!c || !g || !sources) return fail(PBBS_ERR_INVALID, "NULL argument");
-  if (g->ctx->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
   if (nsrc < 1 || nsrc > kMaxSources)
     return fail(PBBS_ERR_INVALID, "source count %d outside [1, %d]", nsrc, kMaxSources);
   for (int i = 0; i < nsrc; ++i)
@@ -1186,7 +1204,7 @@ const void* graph_dev_off(const pbbs_graph* g) { return g->d_off; }
 const int32_t* graph_dev_adj(const pbbs_graph* g) { return g->d_adj; }
 bool graph_off64(const pbbs_graph* g) { return g->off64; }
 int64_t graph_m(const pbbs_graph* g) { return g->m; }
-int graph_device(const pbbs_graph* g) { return g->ctx->device; }
+int graph_device(const pbbs_graph* g) { return g->device; }
 int ctx_num_cus(const pbbs_ctx* c) { return c->num_cus; }
 int64_t graph_n(const pbbs_graph* g) { return g->n; }
 bool graph_symmetric(const pbbs_graph* g) { return g->symmetric; }
@@ -1241,8 +1259,8 @@ extern "C" int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out) {
   out->n = out->m = 0;
   out->offsets = nullptr;
   out->adj = nullptr;
-  HIP_TRY(hipSetDevice(g->ctx->device));
-  HIP_TRY(hipStreamSynchronize(g->ctx->stream));
+  HIP_TRY(hipSetDevice(g->device));
+  if (g->ctx) HIP_TRY(hipStreamSynchronize(g->ctx->stream));
   int64_t* off = static_cast<int64_t*>(std::malloc((g->n + 1) * sizeof(int64_t)));
   int32_t* adj = static_cast<int32_t*>(std::malloc(std::max<int64_t>(g->m, 1) * sizeof(int32_t)));
   if (!off || !adj) {

@@ -182,3 +182,20 @@ def test_invalid_inputs(ctx):
     off[10] = off[11] + 5   # decreasing at 10 -> 11
     with pytest.raises(pb.PbbsError, match="offsets"):
         ctx.upload(pb.HostGraph(off, g.adj))
+
+
+def test_graph_outlives_its_context():
+    """A graph whose uploading context is destroyed stays usable from another
+    context and can still be downloaded and freed."""
+    g = pb.rmat(10, 8 * 2 ** 10, 0.57, 0.19, 0.19, seed=3)
+    want = oracle.serial_bfs(g.offsets, g.adj, 0)
+    c1 = pb.Context(0)
+    dg = c1.upload(g)
+    c1.close()
+    with pb.Context(0) as c2:
+        dg.ctx = c2
+        parent, depth, _ = dg.bfs(0)
+        assert np.array_equal(parent, want[0]) and np.array_equal(depth, want[1])
+        d = dg.download()
+        assert np.array_equal(d.adj, g.adj)
+    dg.free()
