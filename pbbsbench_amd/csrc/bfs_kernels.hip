@@ -1197,9 +1197,14 @@ __device__ void scan_window(const MbfsView<OffT>& a, gptr<const unsigned long lo
     scan_assign(a, r, need, nxt);
     unsigned long long h[kUnroll];
     if (bm) {
+      // the filter words are gathered unconditionally (idle positions read
+      // word 0): a guarded load compiles to a branch with its own wait
+      unsigned fw[kUnroll];
+#pragma unroll
+      for (int s = 0; s < kUnroll; ++s) fw[s] = bm[(cur.ur[s] < 0 ? 0 : cur.ur[s]) >> 5];
 #pragma unroll
       for (int s = 0; s < kUnroll; ++s)
-        if (cur.ur[s] >= 0 && !((bm[cur.ur[s] >> 5] >> (cur.ur[s] & 31)) & 1u)) cur.ur[s] = -1;
+        cur.ur[s] = ((fw[s] >> (cur.ur[s] & 31)) & 1u) ? cur.ur[s] : -1;
     }
 #pragma unroll
     for (int s = 0; s < kUnroll; ++s) {
@@ -1709,9 +1714,15 @@ __global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
     a.fbm[2][w] = 0;
     a.fbm[3][w] = 0;
   }
-  const long long total = (long long)a.nsrc * a.n;
-  for (long long t = tid; t < total; t += nth)   // depth: mbfs_depth_kernel
-    __builtin_nontemporal_store(-1, &a.parent[t]);
+  const long long total = (long long)a.nsrc * a.n;   // depth: mbfs_depth_kernel
+  if ((reinterpret_cast<uintptr_t>(a.parent) & 15) == 0 && (total & 3) == 0) {
+    typedef int i32x4 __attribute__((ext_vector_type(4)));
+    const i32x4 m1 = {-1, -1, -1, -1};
+    i32x4* p4 = reinterpret_cast<i32x4*>(a.parent);
+    for (long long t = tid; t < total / 4; t += nth) __builtin_nontemporal_store(m1, &p4[t]);
+  } else {
+    for (long long t = tid; t < total; t += nth) __builtin_nontemporal_store(-1, &a.parent[t]);
+  }
   for (long long i = tid; i < kMaxGrid; i += nth) a.wg_level[i] = 0;
 }
 
@@ -1780,15 +1791,20 @@ __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
     const long long v0 = q * 4;
     const bool whole = v0 + 4 <= a.n && (a.n & 3) == 0;   // 16-byte aligned rows of depth
+    unsigned long long rvm[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const long long v = v0 + t;
+      rvm[t] = vis[v < a.n ? v : 0] & a.srcmask & (0ull - (unsigned long long)(v < a.n));
+    }
     for (int c = 0; c < 4; ++c) {   // 16 sources of the 4 vertices at a time
-      unsigned long long rvm[4];
       unsigned wv[4][4];
+      // the four rows' bytes are loaded unconditionally (a guarded load
+      // compiles to a branch with its own wait) and only read where reached
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const long long v = v0 + t;
-        rvm[t] = v < a.n ? (vis[v] & a.srcmask) : 0ull;
-        u32x4 x = {0u, 0u, 0u, 0u};
-        if (rvm[t]) x = dl[v * 4 + c];
+        const u32x4 x = dl[(v < a.n ? v : 0) * 4 + c];
         wv[t][0] = x.x;
         wv[t][1] = x.y;
         wv[t][2] = x.z;
