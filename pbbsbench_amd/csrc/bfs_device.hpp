@@ -78,6 +78,8 @@ struct BfsArgs {
   int32_t bu_to_td_div;
   int32_t bu_serial_depth;
   int32_t symmetric;
+  int32_t num_cus;             // CUs of the device (workgroup slot = blockIdx / num_cus)
+  int32_t pad;
 };
 
 // ----------------------------------------------------------- batched BFS
@@ -128,7 +130,7 @@ struct MbfsArgs {
   int32_t nsrc;
   int32_t td_to_bu_div;        // 0 = push levels only
   int32_t pull_div;            // pull when (|F| + edges(F)) * pull_div > m
-  int32_t pad;
+  int32_t num_cus;             // CUs of the device (workgroup slot = blockIdx / num_cus)
 };
 
 }  // namespace pbbs

@@ -198,6 +198,19 @@ __device__ __forceinline__ bool in_set_or_none(gptr<const unsigned> bits, int u)
   return ((w >> (u & 31)) & (unsigned)(u >= 0)) != 0u;   // no short circuit
 }
 
+// Workgroups are dispatched in index order, so the CU slot a workgroup
+// occupies is about blockIdx / num_cus.  A SIMD issues for its oldest
+// wavefront first: with equal static shares the youngest slot falls behind
+// and finishes the level last (measured at rmat24's first bottom-up level:
+// 6.4 us per probe pass in slot 0 against 15.5 us in slot 3).  Raising the
+// issue priority of later slots evens their progress.
+__device__ __forceinline__ void slot_priority(int num_cus) {
+  const unsigned g = num_cus > 0 ? blockIdx.x / (unsigned)num_cus : 0u;
+  if (g == 1) __builtin_amdgcn_s_setprio(1);
+  else if (g == 2) __builtin_amdgcn_s_setprio(2);
+  else if (g >= 3) __builtin_amdgcn_s_setprio(3);
+}
+
 __device__ __forceinline__ long long global_wave() {
   return (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
 }
@@ -924,7 +937,10 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   if (mode == kTopDown)
     top_down(a, sb, stg, E);
   else
+  {
+    slot_priority(A->num_cus);   // bottom-up only: top-down levels measured the same with it
     bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
+  }
   PBBS_STAMP(k, 2);
   stage_flush_block(a, sb, stg);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
@@ -1621,6 +1637,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   a.srcmask = A->srcmask;
   a.k = k;
   int32_t* tile = lds.tile[threadIdx.x / kWave];
+  slot_priority(A->num_cus);
   mbfs_mark(a, cnt);
   Stage stg;
   if (mode == kTopDown)

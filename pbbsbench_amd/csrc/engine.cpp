@@ -670,6 +670,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.bu_to_td_div = c->tuning.bu_to_td_div;
   a.bu_serial_depth = c->tuning.bu_serial_depth;
   a.symmetric = g->symmetric ? 1 : 0;
+  a.num_cus = c->num_cus;
   return a;
 }
 
@@ -1067,6 +1068,7 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.srcmask = nsrc >= 64 ? ~0ull : ((1ull << nsrc) - 1);
   a.td_to_bu_div = c->tuning.td_to_bu_div;
   a.pull_div = c->tuning.batch_pull_div;
+  a.num_cus = c->num_cus;
   pbbs_ctx::Run& r = c->run;
   r.kind = 1;
   r.off64 = g->off64;
