@@ -1045,7 +1045,7 @@ constexpr int kPass = kWave * kUnroll;
 // costs about as much as kRowCost edges (its window slot, the mask and
 // offset loads, the output write-out) whatever its degree, and splitting by
 // edges alone gave the wavefronts over low-degree ranges ~30x the rows.
-constexpr int kRowCost = 16;
+constexpr int kRowCost = 32;   // swept over 8..128 on RMAT 2^22 x 64: 32 best
 // A pull level tests the frontier bitmap before each mask gather when
 // |F_k| < n / kFilterDiv.
 constexpr int kFilterDiv = 4;
