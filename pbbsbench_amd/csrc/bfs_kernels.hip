@@ -549,12 +549,13 @@ __device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine
 }
 
 // Unreached vertices a lane probes per pass of a sorted bottom-up level.  The
-// pass issues every load of all of them (offsets, then row tops, then bitmap
-// words) before it uses any, so a wavefront keeps kBuPer times as many
-// independent row fetches in flight: the first bottom-up level is bound by
-// the latency of one random row-top line per unreached vertex.
+// pass issues every load of all of them (row-top records, or offsets then row
+// tops, then bitmap words) before it uses any, so a wavefront keeps kBuPer
+// times as many independent fetches in flight: the first bottom-up level is
+// bound by the latency of these round trips (3 measured best on RMAT 2^24,
+// against 1 and 2).
 #ifndef PBBS_BU_PER_LANE
-#define PBBS_BU_PER_LANE 2
+#define PBBS_BU_PER_LANE 3
 #endif
 constexpr int kBuPer = PBBS_BU_PER_LANE;
 constexpr int kBuCap = (kBuPer + 1) * kWave;   // queue and park capacity per wavefront
@@ -925,7 +926,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.ctr_cur = glob(st->ctr[c0]);
   // The first bottom-up level after top-down meets almost every vertex
   // unreached; its per-vertex cost grows with the degree, so the shares are
-  // cut at equal sums of 1 + log2(1 + degree).  Later bottom-up levels meet
+  // cut at equal sums of 1 + 2 log2(1 + degree).  Later bottom-up levels meet
   // few, low-degree vertices and split the words evenly.
   a.bu_bounds = glob((prev == kTopDown) ? A->bu_bounds : nullptr);
   a.tops = glob(kTops ? reinterpret_cast<const u32x4*>(A->bu_tops) : nullptr);
@@ -2003,7 +2004,7 @@ __global__ __launch_bounds__(kBlock) void graph_symmetry_kernel(
 }
 
 // ------------------------------------------------------ bottom-up shares
-// Per frontier word: sum over its vertices of degree 0 ? 0 : 1 + floor(log2(1 + deg)).
+// Per frontier word: sum over its vertices of degree 0 ? 0 : 1 + 2 floor(log2(1 + deg)).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void word_weight_kernel(const OffT* __restrict__ off,
                                                              long long n, long long nwords,
@@ -2014,7 +2015,7 @@ __global__ __launch_bounds__(kBlock) void word_weight_kernel(const OffT* __restr
     unsigned x = 0;
     if (v < n) {
       const unsigned long long deg = (unsigned long long)(off[v + 1] - off[v]);
-      if (deg) x = 1u + (63u - (unsigned)__builtin_clzll(deg + 1));
+      if (deg) x = 1u + 2u * (63u - (unsigned)__builtin_clzll(deg + 1));
     }
 #pragma unroll
     for (int d = 1; d < kWave; d <<= 1) x += __shfl_xor(x, d);
