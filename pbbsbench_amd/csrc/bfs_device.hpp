@@ -46,12 +46,13 @@ struct BfsState {
   LevelRecord log[kLevelLogMax];
 };
 
-// Everything a level kernel needs.  The init kernel copies it into device
-// memory so one captured HIP graph of level launches serves every run that
-// uses the same workspace.
 // Single-source level kernel variants: unsorted rows (full-row bottom-up
 // scans), sorted rows read from the CSR, sorted rows with row-top records.
 enum LevelVariant { kLevelUnsorted = 0, kLevelSorted = 1, kLevelSortedTops = 2 };
+
+// Everything a level kernel needs.  The init kernel copies it into device
+// memory so one captured HIP graph of level launches serves every run that
+// uses the same workspace.
 
 template <typename OffT>
 struct BfsArgs {
@@ -66,6 +67,8 @@ struct BfsArgs {
   BfsState* st;
   uint32_t* wg_level;          // per-workgroup level counter (kMaxGrid entries)
   int32_t* host_status;        // pinned host memory: [0] done, [1] levels started
+  BfsState* host_state;        // pinned host memory: the final counters, written by the
+                               // kernel that ends the BFS before it sets host_status[0]
   const uint32_t* bu_bounds;   // first-word index of each wavefront's share in the first
                                // bottom-up level after top-down (num_waves + 1 entries)
   const uint32_t* bu_tops;     // [4n] row-top records {three largest neighbours or ~0,

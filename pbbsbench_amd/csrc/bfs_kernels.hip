@@ -864,6 +864,12 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
       st->reached += z;
       if (z) st->max_depth = k;
       st->finished = 1;
+      BfsState* hst = A->host_state;   // the host reads these once host_status[0] is set
+      hst->reached = st->reached;
+      hst->edges_reached = st->edges_reached;
+      hst->max_depth = st->max_depth;
+      hst->td_levels = st->td_levels;
+      hst->bu_levels = st->bu_levels;
       __threadfence_system();
       __hip_atomic_store(&A->host_status[1], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&A->host_status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);

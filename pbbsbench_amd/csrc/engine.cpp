@@ -661,6 +661,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.st = c->w_state;
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
+  a.host_state = c->host_state;
   a.bu_bounds = bounds;
   a.bu_tops = tops;
   a.n = g->n;
@@ -811,9 +812,8 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
 static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
-  HIP_TRY(hipMemcpyAsync(c->host_state, c->w_state, offsetof(BfsState, log),
-                         hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the kernel that ended the BFS wrote the counters to pinned host_state
+  // (ordered before host_status[0], which the completed event covers)
   std::memset(stats, 0, sizeof *stats);
   stats->levels = c->host_state->max_depth + 1;
   stats->td_levels = c->host_state->td_levels;
