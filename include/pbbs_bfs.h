@@ -74,7 +74,9 @@ typedef struct {
     int32_t grid_blocks;    /* workgroups per level launch; 0 = auto (occupancy x CUs) */
     int32_t bu_serial_depth;/* row entries a lane tests alone (bottom-up) before its
                                wavefront takes over the row; default 32 */
-    int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16 */
+    int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16.  After
+                               the first run on a graph, a run's first chunk is that
+                               graph's previous launch count plus one, capped by this */
     int32_t use_graphs;     /* 1 = replay captured HIP graphs of level launches (default),
                                0 = launch each level kernel on the stream */
     int32_t batch_pull_div; /* batched BFS: pull a level when |F| + sum deg(F) > m / this,

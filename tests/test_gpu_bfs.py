@@ -199,3 +199,22 @@ def test_graph_outlives_its_context():
         d = dg.download()
         assert np.array_equal(d.adj, g.adj)
     dg.free()
+
+
+def test_first_chunk_shorter_than_the_bfs(ctx):
+    """The first level chunk is sized from the graph's previous run: a deeper
+    BFS after a shallow one must continue with further chunks, and a shallow
+    one after a deep one must stop at its own end."""
+    n = 40
+    u = np.arange(n - 1)
+    g = pb.from_edges(n, u, u + 1)                 # path 0 - 1 - ... - 39
+    dg = ctx.upload(g)
+    try:
+        for s in (20, 0, 39, 20, 3):
+            say("path source", s)
+            parent, depth, st = dg.bfs(s)
+            wp, wd = oracle.serial_bfs(g.offsets, g.adj, s)
+            assert np.array_equal(parent, wp) and np.array_equal(depth, wd)
+            assert st["levels"] == int(wd.max()) + 1
+    finally:
+        dg.free()
