@@ -76,7 +76,7 @@ typedef struct {
                                wavefront takes over the row; default 32 */
     int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16.  After
                                the first run on a graph, a run's first chunk is that
-                               graph's previous launch count plus one, capped by this */
+                               graph's previous launch count plus one (at most 1024) */
     int32_t use_graphs;     /* 1 = replay captured HIP graphs of level launches (default),
                                0 = launch each level kernel on the stream */
     int32_t batch_pull_div; /* batched BFS: pull a level when |F| + sum deg(F) > m / this,

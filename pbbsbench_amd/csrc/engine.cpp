@@ -762,10 +762,12 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
 // Launches in a run's first chunk: BFS depths vary little between sources of
 // one graph, so after the first run the chunk covers the last run's launches
 // plus one (a launch past the end costs a few microseconds; one short costs
-// a host round trip), capped by the tuning's chunk_levels.
+// a host round trip), at most kMaxFirstChunk.  A first run starts with the
+// tuning's chunk_levels.
+constexpr int kMaxFirstChunk = 1024;
 static int first_chunk(const pbbs_ctx* c, const pbbs_graph* g, int kind) {
   const int last = g->last_launches[kind];
-  return last > 0 ? std::min(last + 1, c->tuning.chunk_levels) : c->tuning.chunk_levels;
+  return last > 0 ? std::min(last + 1, kMaxFirstChunk) : c->tuning.chunk_levels;
 }
 
 static int level_variant(const pbbs_graph* g, const uint32_t* tops) {
