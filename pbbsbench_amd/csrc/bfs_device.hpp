@@ -99,7 +99,7 @@ struct MbfsState {
   int32_t push_levels;
   int32_t pull_levels;
   int32_t max_depth;
-  unsigned long long reached_pairs;  // (source, vertex) pairs reached (finalise kernel)
+  unsigned long long reached_pairs;  // (source, vertex) pairs reached (summed by the depth pass)
   unsigned long long edges_pairs;    // sum over those pairs of deg(vertex)
 };
 

@@ -1812,6 +1812,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
   const int ns = a.nsrc;
   const int32_t nval = (int32_t)a.n;
   const long long quads = (a.n + 3) / 4;
+  unsigned long long pairs = 0, edges = 0;   // reached (source, vertex) pairs, their degrees
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
     const long long v0 = q * 4;
     const bool whole = v0 + 4 <= a.n && (a.n & 3) == 0;   // 16-byte aligned rows of depth
@@ -1820,6 +1821,15 @@ __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
     for (int t = 0; t < 4; ++t) {
       const long long v = v0 + t;
       rvm[t] = vis[v < a.n ? v : 0] & a.srcmask & (0ull - (unsigned long long)(v < a.n));
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (rvm[t]) {
+        const long long v = v0 + t;
+        const unsigned long long c = (unsigned long long)__popcll(rvm[t]);
+        pairs += c;
+        edges += c * (unsigned long long)(a.off[v + 1] - a.off[v]);
+      }
     }
     for (int c = 0; c < 4; ++c) {   // 16 sources of the 4 vertices at a time
       unsigned wv[4][4];
@@ -1863,23 +1873,12 @@ __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
       }
     }
   }
-}
-
-// Totals over (source, vertex) pairs reached: count and degree sum.
-template <typename OffT>
-__global__ __launch_bounds__(kBlock) void mbfs_stats_kernel(MbfsArgs<OffT> a) {
+  // totals over the reached (source, vertex) pairs: count and degree sum
   __shared__ unsigned long long sp[kWavesPerBlock], se[kWavesPerBlock];
-  unsigned long long pairs = 0, edges = 0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += stride) {
-    const unsigned long long c = (unsigned long long)__popcll(a.vis[v] & a.srcmask);
-    pairs += c;
-    edges += c * (unsigned long long)(a.off[v + 1] - a.off[v]);
-  }
 #pragma unroll
-  for (int d = 1; d < kWave; d <<= 1) {
-    pairs += __shfl_xor(pairs, d);
-    edges += __shfl_xor(edges, d);
+  for (int dd = 1; dd < kWave; dd <<= 1) {
+    pairs += __shfl_xor(pairs, dd);
+    edges += __shfl_xor(edges, dd);
   }
   const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
@@ -1893,10 +1892,13 @@ __global__ __launch_bounds__(kBlock) void mbfs_stats_kernel(MbfsArgs<OffT> a) {
       P += sp[i];
       Ed += se[i];
     }
-    atomicAdd(&a.st->reached_pairs, P);
-    atomicAdd(&a.st->edges_pairs, Ed);
+    if (P) {
+      atomicAdd(&a.st->reached_pairs, P);
+      atomicAdd(&a.st->edges_pairs, Ed);
+    }
   }
 }
+
 
 // ---------------------------------------------------------- graph metadata
 // counters: [0] isolated vertices, [1] max degree, [2] bad offsets (decreasing
@@ -2273,12 +2275,6 @@ hipError_t launch_mbfs_depth(const MbfsArgs<OffT>& a, int grid, hipStream_t stre
   return hipGetLastError();
 }
 
-template <typename OffT>
-hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream) {
-  hipLaunchKernelGGL(mbfs_stats_kernel<OffT>, dim3(grid), dim3(kBlock), 0, stream, a);
-  return hipGetLastError();
-}
-
 #ifdef PBBS_WAVE_TIMING
 extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
   const long long cap = (long long)kTimingLevels * kTimingWaves * 3;
@@ -2310,7 +2306,6 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
   template hipError_t build_degree_ranks<OffT>(const OffT*, const int32_t*, long long, long long, \
                                                int32_t*, int32_t*, int32_t*, int32_t*,           \
                                                long long*, int, hipStream_t);                    \
-  template hipError_t launch_mbfs_stats<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
   template hipError_t launch_mbfs_depth<OffT>(const MbfsArgs<OffT>&, int, hipStream_t);          \
   template hipError_t launch_word_weights<OffT>(const OffT*, long long, unsigned long long*, int, \
                                                hipStream_t);                                    \

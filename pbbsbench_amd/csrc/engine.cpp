@@ -52,8 +52,6 @@ hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, 
                               int32_t* rank, int32_t* inv_rank, int32_t* adj_rank, int32_t* rows,
                               long long* nrows, int grid, hipStream_t stream);
 template <typename OffT>
-hipError_t launch_mbfs_stats(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
-template <typename OffT>
 hipError_t launch_mbfs_depth(const MbfsArgs<OffT>& a, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_word_weights(const OffT* off, long long n, unsigned long long* ww, int grid,
@@ -137,6 +135,7 @@ struct pbbs_ctx {
   int64_t cap_bn = 0;                    // vertices the mask buffers hold
   unsigned long long* b_masks = nullptr; // vis | fr0 | fr1 | fr2 | fr3
   MbfsState* b_state = nullptr;
+  MbfsState* b_host_state = nullptr;     // pinned: the control block after the depth pass
   void* b_args = nullptr;                // device copy of MbfsArgs<OffT>
   int32_t* b_sources = nullptr;          // kMaxSources
   int64_t cap_bout = 0;                  // int32 entries of each output buffer
@@ -271,6 +270,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_state);
+  if (c->b_host_state) (void)hipHostFree(c->b_host_state);
   (void)hipFree(c->b_args);
   (void)hipFree(c->b_sources);
   (void)hipFree(c->b_parent);
@@ -742,6 +742,8 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
       else
         HIP_TRY(launch_mbfs_depth<uint32_t>(
             *reinterpret_cast<MbfsArgs<uint32_t>*>(c->b_last_args), grid, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->b_host_state, c->b_state, sizeof(MbfsState),
+                             hipMemcpyDeviceToHost, c->stream));
       HIP_TRY(hipEventRecord(c->ev_end, c->stream));
       r.finalized = true;
       if (!wait) return PBBS_OK;
@@ -972,6 +974,7 @@ extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
 static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, bool need_out) {
   if (!c->b_state) {
     HIP_TRY(hipMalloc(&c->b_state, sizeof(MbfsState)));
+    HIP_TRY(hipHostMalloc((void**)&c->b_host_state, sizeof(MbfsState), hipHostMallocDefault));
     HIP_TRY(hipMalloc(&c->b_args, sizeof(MbfsArgs<int64_t>)));
     HIP_TRY(hipMalloc(&c->b_sources, kMaxSources * sizeof(int32_t)));
   }
@@ -1114,15 +1117,8 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
 static int batch_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
-  if (c->b_last_args_off64)
-    HIP_TRY(launch_mbfs_stats<int64_t>(*reinterpret_cast<MbfsArgs<int64_t>*>(c->b_last_args),
-                                       c->num_cus * 4, c->stream));
-  else
-    HIP_TRY(launch_mbfs_stats<uint32_t>(*reinterpret_cast<MbfsArgs<uint32_t>*>(c->b_last_args),
-                                        c->num_cus * 4, c->stream));
-  MbfsState hst;
-  HIP_TRY(hipMemcpyAsync(&hst, c->b_state, sizeof hst, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  // the depth pass summed the totals; its copy to pinned memory precedes ev_end
+  const MbfsState& hst = *c->b_host_state;
   std::memset(stats, 0, sizeof *stats);
   stats->levels = hst.max_depth + 1;
   stats->td_levels = hst.push_levels;
