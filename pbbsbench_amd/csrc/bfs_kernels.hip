@@ -1370,6 +1370,11 @@ __device__ __forceinline__ unsigned find_entry(gptr<const uint32_t> qe, unsigned
   return lo + (unsigned)__popcll(__ballot(pr)) - 1;
 }
 
+#ifndef PBBS_PUSH_PER_LANE
+#define PBBS_PUSH_PER_LANE 2
+#endif
+constexpr int kPushPer = PBBS_PUSH_PER_LANE;   // edges per lane per push step
+
 template <typename OffT>
 __device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg,
                           unsigned long long E) {
@@ -1406,43 +1411,60 @@ __device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg
       }
       const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : (uint32_t)Es;
       const uint32_t lim = min(wend, ee);
-      for (uint32_t base = e; base < lim; base += kWave) {
-        const uint32_t my = base + lane;
-        const bool valid = (unsigned long long)base + lane < (unsigned long long)lim;
-        int j = 0;
+      // kPushPer edges per lane per step: every step issues the loads of all
+      // of them (adjacency, then visited and frontier masks) before it uses
+      // any, since a step is a chain of dependent round trips.
+      for (uint32_t base = e; base < lim; base += kPushPer * kWave) {
+        bool valid[kPushPer];
+        int v[kPushPer], rv[kPushPer], uj[kPushPer];
+        unsigned long long um[kPushPer];
 #pragma unroll
-        for (int st = kWave / 2; st >= 1; st >>= 1) {
-          const uint32_t qt = __shfl(qo, j + st);
-          if (qt <= my) j += st;
+        for (int t = 0; t < kPushPer; ++t) {
+          const uint32_t my = base + (uint32_t)(t * kWave) + lane;
+          valid[t] = (unsigned long long)base + t * kWave + lane < (unsigned long long)lim;
+          int j = 0;
+#pragma unroll
+          for (int st = kWave / 2; st >= 1; st >>= 1) {
+            const uint32_t qt = __shfl(qo, j + st);
+            if (qt <= my) j += st;
+          }
+          const OffT rsj = shfl_off(rs, j);
+          const uint32_t qoj = __shfl(qo, j);
+          um[t] = __shfl(qm, j);
+          uj[t] = __shfl(qv, j);
+          // idle lanes read entry 0 (m >= 1 whenever a push level has edges;
+          // upload checked every entry is a vertex id) and are masked below
+          const OffT ed = valid[t] ? rsj + (OffT)(my - qoj) : (OffT)0;
+          v[t] = a.adj[ed];
+          rv[t] = a.adj_rank[ed];
         }
-        const OffT rsj = shfl_off(rs, j);
-        const uint32_t qoj = __shfl(qo, j);
-        const unsigned long long um = __shfl(qm, j);
-        const int uj = __shfl(qv, j);
-        bool fresh = false;
-        int v = 0;
-        if (valid) {
-          const OffT e = rsj + (OffT)(my - qoj);
-          v = a.adj[e];
-          const int rv = a.adj_rank[e];
-          const unsigned long long nb = um & ~(a.vis[v] | a.fr_cur[rv]) & a.srcmask;
+        unsigned long long nbv[kPushPer];
+#pragma unroll
+        for (int t = 0; t < kPushPer; ++t) {
+          const unsigned long long seen = a.vis[v[t]] | a.fr_cur[rv[t]];
+          nbv[t] = um[t] & ~seen & a.srcmask & (0ull - (unsigned long long)valid[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < kPushPer; ++t) {
+          bool fresh = false;
+          const unsigned long long nb = nbv[t];
           if (nb) {
-            const unsigned long long old = atomicOr(gen(&a.fr_next[rv]), nb);
+            const unsigned long long old = atomicOr(gen(&a.fr_next[rv[t]]), nb);
             fresh = old == 0ull;
-            if (fresh) atomicOr(gen(&a.bm_next[rv >> 5]), 1u << (rv & 31));
+            if (fresh) atomicOr(gen(&a.bm_next[rv[t] >> 5]), 1u << (rv[t] & 31));
             // every frontier edge into a new bit competes; the max stays
             for (unsigned long long b = nb; b; b &= b - 1) {
               const int i = __builtin_ctzll(b);
-              const long long o = (long long)i * a.n + v;
-              atomicMax(gen(&a.parent[o]), uj);
-              if (!((old >> i) & 1ull)) put_depth(a, i, v, a.k + 1);   // the bit's first setter
+              const long long o = (long long)i * a.n + v[t];
+              atomicMax(gen(&a.parent[o]), uj[t]);
+              if (!((old >> i) & 1ull)) put_depth(a, i, v[t], a.k + 1);   // the bit's first setter
             }
           }
-        }
-        if (__ballot(fresh)) {
-          uint32_t deg = 0;
-          if (fresh) deg = (uint32_t)(a.off[v + 1] - a.off[v]);
-          stage_push(a, sb, stg, fresh, v, deg);
+          if (__ballot(fresh)) {
+            uint32_t deg = 0;
+            if (fresh) deg = (uint32_t)(a.off[v[t] + 1] - a.off[v[t]]);
+            stage_push(a, sb, stg, fresh, v[t], deg);
+          }
         }
       }
       e = lim;
