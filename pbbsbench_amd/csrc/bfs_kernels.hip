@@ -396,22 +396,24 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
       const uint32_t qoj = __shfl(qo, j);
       bool fresh = false;
       int v = 0;
+      uint32_t deg = 0;
       if (valid) {
         v = a.adj[rsj + (OffT)(my - qoj)];
-        if (a.depth[v] > k) {
+        // the row bounds load with the depth (one round trip, not two after a claim)
+        const int dv = a.depth[v];
+        const OffT o0 = a.off[v], o1 = a.off[v + 1];
+        if (dv > k) {
           const int old = atomicMax(gen(&a.parent[v]), u);
           if (old == -1) {
             fresh = true;
             a.depth[v] = k + 1;
+            deg = (uint32_t)(o1 - o0);
           }
         }
       }
       if (__ballot(fresh)) {
-        uint32_t deg = 0;
-        if (fresh) {
-          deg = (uint32_t)(a.off[v + 1] - a.off[v]);
+        if (fresh)
           atomicOr(reinterpret_cast<unsigned*>(gen(a.fb_next)) + (v >> 5), 1u << (v & 31));
-        }
         const unsigned long long z = __ballot(fresh && deg == 0);
         if (z && lane == 0) atomicAdd(gen(a.d0_next), (unsigned)__popcll(z));
         stage_push(a, sb, stg, fresh && deg > 0, v, deg);
