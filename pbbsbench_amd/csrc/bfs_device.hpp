@@ -82,7 +82,7 @@ struct BfsArgs {
   int32_t bu_serial_depth;
   int32_t symmetric;
   int32_t num_cus;             // CUs of the device (workgroup slot = blockIdx / num_cus)
-  int32_t pad;
+  int32_t small_rows;          // 1 if no row has more than 64 entries (top-down slices by entry)
 };
 
 // ----------------------------------------------------------- batched BFS

@@ -240,7 +240,8 @@ struct LevelView {
   gptr<uint32_t> d0_next;
   gptr<const unsigned long long> ctr_cur;  // F_k's counters (all sub-queues)
   gptr<const uint32_t> bu_bounds;       // weighted wavefront shares, or null for equal shares
-  gptr<const u32x4> tops;               // per vertex {three largest neighbours, degree}, or null
+  gptr<const u32x4> tops;               // per vertex {six largest neighbours, row end, degree}, or null
+  int small_rows;                       // every row has at most 64 entries
   long long n;
   long long nwords;
   long long qcap;
@@ -349,21 +350,23 @@ __device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const S
 template <typename OffT>
 __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                          gptr<const int32_t> q, gptr<const uint32_t> qe,
-                         unsigned nf, uint32_t E, uint32_t eb, uint32_t ee) {
+                         unsigned nf, uint32_t E, uint32_t eb, uint32_t ee, int start) {
   const unsigned lane = lane_id();
   const int k = a.k;
-  // Largest i with qe[i] <= eb (qe[0] == 0 and qe is strictly increasing).
-  unsigned lo = 0, hi = nf;
-  while (hi - lo > (unsigned)kWave) {
-    const unsigned step = (hi - lo + kWave - 1) / kWave;
-    const unsigned idx = lo + lane * step;
-    const bool p = idx < hi && qe[idx] <= eb;
-    const unsigned cnt = (unsigned)__popcll(__ballot(p));
-    lo += (cnt - 1) * step;
-    hi = min(hi, lo + step);
-  }
   unsigned i;
-  {
+  if (start >= 0) {
+    i = (unsigned)start;   // eb is entry start's first edge
+  } else {
+    // Largest i with qe[i] <= eb (qe[0] == 0 and qe is strictly increasing).
+    unsigned lo = 0, hi = nf;
+    while (hi - lo > (unsigned)kWave) {
+      const unsigned step = (hi - lo + kWave - 1) / kWave;
+      const unsigned idx = lo + lane * step;
+      const bool p = idx < hi && qe[idx] <= eb;
+      const unsigned cnt = (unsigned)__popcll(__ballot(p));
+      lo += (cnt - 1) * step;
+      hi = min(hi, lo + step);
+    }
     const unsigned idx = lo + lane;
     const bool p = idx < hi && qe[idx] <= eb;
     i = lo + (unsigned)__popcll(__ballot(p)) - 1;
@@ -431,6 +434,31 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                          unsigned long long E) {
   const unsigned long long W = (unsigned long long)num_waves();
   const unsigned long long gw = (unsigned long long)global_wave();
+  if (a.small_rows) {
+    // Every row has at most 64 entries: slices of equal entry counts are
+    // within one window of balanced, and a slice's first entry is known, so
+    // the search over the degree prefix is skipped.
+    unsigned long long N = 0;
+    for (int s = 0; s < kSubQueues; ++s) N += (uint32_t)a.ctr_cur[s];
+    const unsigned long long ib = N * gw / W, ie = N * (gw + 1) / W;
+    if (ib >= ie) return;
+    unsigned long long P = 0;
+    for (int s = 0; s < kSubQueues; ++s) {
+      const unsigned long long c = a.ctr_cur[s];
+      const unsigned nf = (unsigned)c;
+      const unsigned long long lo = ib > P ? ib : P;
+      const unsigned long long hi = ie < P + nf ? ie : P + nf;
+      if (lo < hi) {
+        const gptr<const uint32_t> qe = a.qe_cur + s * a.qcap;
+        const unsigned i0 = (unsigned)(lo - P), i1 = (unsigned)(hi - P);
+        const uint32_t Es = (uint32_t)(c >> 32);
+        const uint32_t eb = qe[i0], ee = i1 < nf ? qe[i1] : Es;
+        td_range(a, sb, stg, a.q_cur + s * a.qcap, qe, nf, Es, eb, ee, (int)i0);
+      }
+      P += nf;
+    }
+    return;
+  }
   const unsigned long long eb = E * gw / W, ee = E * (gw + 1) / W;
   if (eb >= ee) return;
   unsigned long long P = 0;
@@ -441,7 +469,7 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     const unsigned long long hi = ee < P + Es ? ee : P + Es;
     if (lo < hi)
       td_range(a, sb, stg, a.q_cur + s * a.qcap, a.qe_cur + s * a.qcap, (unsigned)c,
-               (uint32_t)Es, (uint32_t)(lo - P), (uint32_t)(hi - P));
+               (uint32_t)Es, (uint32_t)(lo - P), (uint32_t)(hi - P), -1);
     P += Es;
   }
 }
@@ -938,6 +966,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   // few, low-degree vertices and split the words evenly.
   a.bu_bounds = glob((prev == kTopDown) ? A->bu_bounds : nullptr);
   a.tops = glob(kTops ? reinterpret_cast<const u32x4*>(A->bu_tops) : nullptr);
+  a.small_rows = A->small_rows;
   a.n = A->n;
   a.nwords = A->nwords;
   a.k = k;

@@ -672,6 +672,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.bu_serial_depth = c->tuning.bu_serial_depth;
   a.symmetric = g->symmetric ? 1 : 0;
   a.num_cus = c->num_cus;
+  a.small_rows = g->max_degree <= kWave ? 1 : 0;
   return a;
 }
 
