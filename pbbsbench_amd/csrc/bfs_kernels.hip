@@ -316,9 +316,22 @@ __device__ __forceinline__ void stage_push(const View& a, SB& sb, Stage& s, bool
   s.deg += total;
 }
 
-// End of level: one reservation for the whole workgroup, then copy-out.
+// Bottom-up levels only count their finds (count, degree sum): F_{k+1} is
+// their bitmap, and the level after one reads that bitmap, so no queue
+// entry is staged.
+__device__ __forceinline__ void stage_count(Stage& s, bool take, uint32_t deg) {
+  const unsigned long long mask = __ballot(take);
+  if (!mask) return;
+  const unsigned incl = wave_inclusive_sum(take ? deg : 0u);
+  s.cnt += (unsigned)__popcll(mask);
+  s.deg += (unsigned)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+}
+
+// End of level: one reservation for the whole workgroup, then copy-out
+// (none when the level only counted).
 template <typename View, typename SB>
-__device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const Stage& s) {
+__device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const Stage& s,
+                                                  bool staged = true) {
   const int w = threadIdx.x / kWave;
   if (lane_id() == 0) {
     sb.cnt[w] = s.cnt;
@@ -339,7 +352,7 @@ __device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const S
     qb += sb.cnt[i];
     eb += sb.deg[i];
   }
-  stage_copy_out(a, sb, w, s.cnt, qb, eb);
+  if (staged) stage_copy_out(a, sb, w, s.cnt, qb, eb);
 }
 
 // ------------------------------------------------------------------ top-down
@@ -347,12 +360,60 @@ __device__ __forceinline__ void stage_flush_block(const View& a, SB& sb, const S
 // find the entry holding edge eb with a 64-ary search over qe, then walk
 // windows of 64 entries; lane l of a batch takes edge base+l and finds its
 // source by a register binary search over the window's qe values.
+// Edges [e, lim) of a window of up to 64 frontier entries held one per lane:
+// vertex qv, exclusive degree prefix qo (~0 past the window's entries), row
+// start rs.  Lane l takes edge base + l and finds its entry by a register
+// binary search over qo; a claim that returns -1 makes the neighbour new.
+template <typename OffT>
+__device__ __forceinline__ void td_window(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                                          int32_t qv, uint32_t qo, OffT rs, uint32_t e,
+                                          uint32_t lim) {
+  const unsigned lane = lane_id();
+  const int k = a.k;
+  for (uint32_t base = e; base < lim; base += kWave) {
+    const uint32_t my = base + lane;  // < lim + 64 <= 2^32 - 1 + 64: compare in 64 bits
+    const bool valid = (unsigned long long)base + lane < (unsigned long long)lim;
+    int j = 0;
+#pragma unroll
+    for (int s = kWave / 2; s >= 1; s >>= 1) {
+      const uint32_t qt = __shfl(qo, j + s);
+      if (qt <= my) j += s;
+    }
+    const int u = __shfl(qv, j);
+    const OffT rsj = shfl_off(rs, j);
+    const uint32_t qoj = __shfl(qo, j);
+    bool fresh = false;
+    int v = 0;
+    uint32_t deg = 0;
+    if (valid) {
+      v = a.adj[rsj + (OffT)(my - qoj)];
+      // the row bounds load with the depth (one round trip, not two after a claim)
+      const int dv = a.depth[v];
+      const OffT o0 = a.off[v], o1 = a.off[v + 1];
+      if (dv > k) {
+        const int old = atomicMax(gen(&a.parent[v]), u);
+        if (old == -1) {
+          fresh = true;
+          a.depth[v] = k + 1;
+          deg = (uint32_t)(o1 - o0);
+        }
+      }
+    }
+    if (__ballot(fresh)) {
+      if (fresh)
+        atomicOr(reinterpret_cast<unsigned*>(gen(a.fb_next)) + (v >> 5), 1u << (v & 31));
+      const unsigned long long z = __ballot(fresh && deg == 0);
+      if (z && lane == 0) atomicAdd(gen(a.d0_next), (unsigned)__popcll(z));
+      stage_push(a, sb, stg, fresh && deg > 0, v, deg);
+    }
+  }
+}
+
 template <typename OffT>
 __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                          gptr<const int32_t> q, gptr<const uint32_t> qe,
                          unsigned nf, uint32_t E, uint32_t eb, uint32_t ee, int start) {
   const unsigned lane = lane_id();
-  const int k = a.k;
   unsigned i;
   if (start >= 0) {
     i = (unsigned)start;   // eb is entry start's first edge
@@ -385,43 +446,7 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     }
     const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : E;
     const uint32_t lim = min(wend, ee);
-    for (uint32_t base = e; base < lim; base += kWave) {
-      const uint32_t my = base + lane;  // < lim + 64 <= 2^32 - 1 + 64: compare in 64 bits
-      const bool valid = (unsigned long long)base + lane < (unsigned long long)lim;
-      int j = 0;
-#pragma unroll
-      for (int s = kWave / 2; s >= 1; s >>= 1) {
-        const uint32_t qt = __shfl(qo, j + s);
-        if (qt <= my) j += s;
-      }
-      const int u = __shfl(qv, j);
-      const OffT rsj = shfl_off(rs, j);
-      const uint32_t qoj = __shfl(qo, j);
-      bool fresh = false;
-      int v = 0;
-      uint32_t deg = 0;
-      if (valid) {
-        v = a.adj[rsj + (OffT)(my - qoj)];
-        // the row bounds load with the depth (one round trip, not two after a claim)
-        const int dv = a.depth[v];
-        const OffT o0 = a.off[v], o1 = a.off[v + 1];
-        if (dv > k) {
-          const int old = atomicMax(gen(&a.parent[v]), u);
-          if (old == -1) {
-            fresh = true;
-            a.depth[v] = k + 1;
-            deg = (uint32_t)(o1 - o0);
-          }
-        }
-      }
-      if (__ballot(fresh)) {
-        if (fresh)
-          atomicOr(reinterpret_cast<unsigned*>(gen(a.fb_next)) + (v >> 5), 1u << (v & 31));
-        const unsigned long long z = __ballot(fresh && deg == 0);
-        if (z && lane == 0) atomicAdd(gen(a.d0_next), (unsigned)__popcll(z));
-        stage_push(a, sb, stg, fresh && deg > 0, v, deg);
-      }
-    }
+    td_window(a, sb, stg, qv, qo, rs, e, lim);
     e = lim;
     i += kWave;
   }
@@ -590,6 +615,7 @@ __device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine
 constexpr int kBuPer = PBBS_BU_PER_LANE;
 constexpr int kBuCap = (kBuPer + 1) * kWave;   // queue and park capacity per wavefront
 constexpr int kTopEntries = 6;                 // neighbours held by a row-top record
+static_assert(kBuCap >= 2 * kWave, "the bottom-up and bitmap top-down lists take 64 + 63 entries");
 
 template <typename OffT>
 struct BuLds {
@@ -601,7 +627,7 @@ struct BuLds {
 
 // Records the found lanes: parent/depth, their F_{k+1} bits (one atomicOr per
 // distinct word; lanes hold ascending ids so a word's lanes are adjacent),
-// and the frontier append.
+// and their count and degree sum.
 template <typename OffT>
 __device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                                           bool valid, bool nw, int v, int found, uint32_t deg) {
@@ -623,7 +649,7 @@ __device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb
     const bool tail = lane == (unsigned)(kWave - 1) || ((heads >> (lane + 1)) & 1ull);
     if (tail && bits) atomicOr(gen(&a.fb_next[key]), bits);
   }
-  stage_push(a, sb, stg, nw, v, deg);
+  stage_count(stg, nw, deg);
 }
 
 template <typename OffT>
@@ -859,6 +885,65 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
   if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark, kTops ? kTopEntries : 4);
 }
 
+// A top-down level after a bottom-up one: F_k exists only as its bitmap.
+// Each wavefront takes an equal share of the bitmap words, gathers its
+// frontier vertices 64 at a time (one per lane, with row start and degree)
+// and expands their edges in windows.  Such frontiers are small and of low
+// degree (the switch back needs |F| <= n / bu_to_td_div).
+template <typename OffT>
+__device__ void td_list(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
+                        unsigned cnt) {
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  wave_lds_sync();
+  const bool mine = lane < cnt;
+  const int u = mine ? L.q[w][lane] : 0;
+  OffT rs = 0;
+  uint32_t deg = 0;
+  if (mine) {
+    rs = a.off[u];
+    deg = (uint32_t)(a.off[u + 1] - rs);
+  }
+  const uint32_t incl = wave_inclusive_sum(deg);
+  const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)incl, kWave - 1);
+  td_window(a, sb, stg, u, mine ? incl - deg : 0xffffffffu, rs, 0u, total);
+}
+
+template <typename OffT>
+__device__ void td_from_bitmap(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
+                               BuLds<OffT>& L) {
+  const unsigned lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  const gptr<const unsigned long long> fb = (gptr<const unsigned long long>)a.fb_cur;
+  const long long W = num_waves();
+  const long long gw = global_wave();
+  const long long w0 = a.nwords * gw / W, w1 = a.nwords * (gw + 1) / W;
+  unsigned nq = 0;
+  for (long long g0 = w0; g0 < w1; g0 += kWave) {
+    const long long mb = g0 + lane;
+    const unsigned long long word = mb < w1 ? fb[mb] : 0ull;
+    unsigned long long todo = __ballot(word != 0ull);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const unsigned long long wd = __shfl(word, j);
+      // nq < 64 here and the list holds kBuCap >= 128 entries
+      if ((wd >> lane) & 1ull) L.q[w][nq + lanes_below(wd)] = (int)((g0 + j) * kWave + lane);
+      nq += (unsigned)__popcll(wd);
+      if (nq >= (unsigned)kWave) {
+        td_list(a, sb, stg, L, kWave);
+        wave_lds_sync();
+        const unsigned rest = nq - kWave;
+        const int tmp = lane < rest ? L.q[w][kWave + lane] : 0;
+        wave_lds_sync();
+        if (lane < rest) L.q[w][lane] = tmp;
+        nq = rest;
+      }
+    }
+  }
+  if (nq) td_list(a, sb, stg, L, nq);
+}
+
 // -------------------------------------------------------------- level driver
 template <typename OffT>
 __device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsigned long long cnt,
@@ -972,15 +1057,18 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.k = k;
   a.serial_depth = A->bu_serial_depth;
   Stage stg;
-  if (mode == kTopDown)
-    top_down(a, sb, stg, E);
-  else
+  if (mode == kTopDown) {
+    if (prev == kBottomUp)
+      td_from_bitmap(a, sb, stg, bl);
+    else
+      top_down(a, sb, stg, E);
+  } else
   {
     slot_priority(A->num_cus);   // bottom-up only: top-down levels measured the same with it
     bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
   }
   PBBS_STAMP(k, 2);
-  stage_flush_block(a, sb, stg);
+  stage_flush_block(a, sb, stg, mode == kTopDown);
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
 }
 
