@@ -1946,70 +1946,84 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
 // per wavefront.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
+  // A workgroup takes tiles of 256 vertices: their 64-byte depth rows are
+  // staged in LDS with coalesced 16-byte loads (row stride 17 dwords), then
+  // wavefront w writes sources [16w, 16w + 16) for the tile, lane l taking
+  // vertices 4l .. 4l + 3 (16-byte stores, 1 KB contiguous per source).
+  constexpr int kRowDw = 17;
+  static_assert(kWavesPerBlock * 16 == kMaxSources, "four wavefronts of 16 sources each");
+  __shared__ uint32_t rows[kBlock * kRowDw];
+  __shared__ unsigned long long rvm_lds[kBlock];
   const gptr<const unsigned long long> vis = glob((const unsigned long long*)a.vis);
   const gptr<const u32x4> dl = glob((const u32x4*)a.dlev);
   const gptr<int32_t> depth = glob(a.depth);
-  const long long stride = (long long)gridDim.x * blockDim.x;
   const int ns = a.nsrc;
   const int32_t nval = (int32_t)a.n;
-  const long long quads = (a.n + 3) / 4;
+  const unsigned lane = lane_id();
+  const int wv = threadIdx.x / kWave;
+  const long long tiles = (a.n + kBlock - 1) / kBlock;
   unsigned long long pairs = 0, edges = 0;   // reached (source, vertex) pairs, their degrees
-  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < quads; q += stride) {
-    const long long v0 = q * 4;
-    const bool whole = v0 + 4 <= a.n && (a.n & 3) == 0;   // 16-byte aligned rows of depth
-    unsigned long long rvm[4];
+  for (long long tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const long long vb = tile * kBlock;
+    __syncthreads();   // the previous tile's readers are done with the LDS rows
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const long long v = v0 + t;
-      rvm[t] = vis[v < a.n ? v : 0] & a.srcmask & (0ull - (unsigned long long)(v < a.n));
+    for (int r = 0; r < 4; ++r) {
+      const int c = r * kBlock + (int)threadIdx.x;   // 16-byte chunk c of the tile
+      const long long v = vb + c / 4;
+      const u32x4 x = dl[(v < a.n ? v : 0) * 4 + (c & 3)];
+      uint32_t* d = &rows[(c / 4) * kRowDw + (c & 3) * 4];
+      d[0] = x.x;
+      d[1] = x.y;
+      d[2] = x.z;
+      d[3] = x.w;
     }
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      if (rvm[t]) {
-        const long long v = v0 + t;
-        const unsigned long long c = (unsigned long long)__popcll(rvm[t]);
+    {
+      const long long v = vb + threadIdx.x;
+      const unsigned long long m =
+          vis[v < a.n ? v : 0] & a.srcmask & (0ull - (unsigned long long)(v < a.n));
+      rvm_lds[threadIdx.x] = m;
+      if (m) {
+        const unsigned long long c = (unsigned long long)__popcll(m);
         pairs += c;
         edges += c * (unsigned long long)(a.off[v + 1] - a.off[v]);
       }
     }
-    for (int c = 0; c < 4; ++c) {   // 16 sources of the 4 vertices at a time
-      unsigned wv[4][4];
-      // the four rows' bytes are loaded unconditionally (a guarded load
-      // compiles to a branch with its own wait) and only read where reached
+    __syncthreads();
+    const int vl = (int)lane * 4;
+    const long long v0 = vb + vl;
+    const bool whole = v0 + 4 <= a.n && (a.n & 3) == 0;   // 16-byte aligned rows of depth
+    unsigned long long rvm[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const long long v = v0 + t;
-        const u32x4 x = dl[(v < a.n ? v : 0) * 4 + c];
-        wv[t][0] = x.x;
-        wv[t][1] = x.y;
-        wv[t][2] = x.z;
-        wv[t][3] = x.w;
-      }
+    for (int t = 0; t < 4; ++t) rvm[t] = rvm_lds[vl + t];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+    for (int jd = 0; jd < 4; ++jd) {   // the wavefront's 16 sources, one dword of 4 at a time
+      const int i0 = 16 * wv + 4 * jd;
+      if (i0 >= ns) break;
+      uint32_t word[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const int i = 16 * c + 4 * j + b;
-          if (i >= ns) break;
-          int32_t d[4];
-          bool keep[4];
+      for (int t = 0; t < 4; ++t) word[t] = rows[(vl + t) * kRowDw + i0 / 4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const unsigned lv = (wv[t][j] >> (8 * b)) & 0xFFu;
-            const bool hit = (rvm[t] >> i) & 1ull;
-            d[t] = hit ? (int32_t)lv : nval;
-            keep[t] = hit && lv == (unsigned)kDepthEscape;   // written during the levels
-          }
-          const long long o = (long long)i * a.n + v0;
-          if (whole && !(keep[0] | keep[1] | keep[2] | keep[3])) {
-            typedef int i32x4 __attribute__((ext_vector_type(4)));
-            const i32x4 dv = {d[0], d[1], d[2], d[3]};
-            __builtin_nontemporal_store(dv, (gptr<i32x4>)(depth + o));
-          } else {
+      for (int b = 0; b < 4; ++b) {
+        const int i = i0 + b;
+        if (i >= ns) break;
+        int32_t d[4];
+        bool keep[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t)
-              if (v0 + t < a.n && !keep[t]) depth[o + t] = d[t];
-          }
+        for (int t = 0; t < 4; ++t) {
+          const unsigned lv = (word[t] >> (8 * b)) & 0xFFu;
+          const bool hit = (rvm[t] >> i) & 1ull;
+          d[t] = hit ? (int32_t)lv : nval;
+          keep[t] = hit && lv == (unsigned)kDepthEscape;   // written during the levels
+        }
+        const long long o = (long long)i * a.n + v0;
+        if (whole && !(keep[0] | keep[1] | keep[2] | keep[3])) {
+          typedef int i32x4 __attribute__((ext_vector_type(4)));
+          const i32x4 dv = {d[0], d[1], d[2], d[3]};
+          __builtin_nontemporal_store(dv, (gptr<i32x4>)(depth + o));
+        } else {
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (v0 + t < a.n && !keep[t]) depth[o + t] = d[t];
         }
       }
     }
@@ -2021,10 +2035,9 @@ __global__ __launch_bounds__(kBlock) void mbfs_depth_kernel(MbfsArgs<OffT> a) {
     pairs += __shfl_xor(pairs, dd);
     edges += __shfl_xor(edges, dd);
   }
-  const int w = threadIdx.x / kWave;
-  if (lane_id() == 0) {
-    sp[w] = pairs;
-    se[w] = edges;
+  if (lane == 0) {
+    sp[wv] = pairs;
+    se[wv] = edges;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
