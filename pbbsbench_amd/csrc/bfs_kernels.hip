@@ -1174,12 +1174,18 @@ constexpr int kPass = kWave * kUnroll;
 constexpr int kRowCost = 32;   // swept over 8..128 on RMAT 2^22 x 64: 32 best
 // A pull level tests the frontier bitmap before each mask gather when
 // |F_k| < n / kFilterDiv.
-constexpr int kFilterDiv = 4;
+#ifndef PBBS_FILTER_DIV
+#define PBBS_FILTER_DIV 4
+#endif
+constexpr int kFilterDiv = PBBS_FILTER_DIV;
 // When the previous pull left fewer than nrows / kFewRowsDiv rows with
 // sources to find, a pull level splits whole rows evenly over the wavefronts:
 // most rows are then skipped, at a cost per row, and an edge-weighted split
 // piles the many rows of low-degree id ranges on a few wavefronts.
-constexpr int kFewRowsDiv = 4;
+#ifndef PBBS_FEW_ROWS_DIV
+#define PBBS_FEW_ROWS_DIV 4
+#endif
+constexpr int kFewRowsDiv = PBBS_FEW_ROWS_DIV;
 
 template <typename OffT>
 struct MbfsView {
