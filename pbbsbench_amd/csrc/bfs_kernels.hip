@@ -1902,9 +1902,13 @@ __global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, MbfsArgs<OffT>* dst) {
   __shared__ int first[kMaxSources];
+  __shared__ int src[kMaxSources];
+  __shared__ unsigned long long deg[kMaxSources];
   const int t = threadIdx.x;
   if (t < a.nsrc) {
     const int s = a.sources[t];
+    src[t] = s;
+    deg[t] = (unsigned long long)(a.off[s + 1] - a.off[s]);
     const unsigned long long bit = 1ull << t;
     a.parent[(long long)t * a.n + s] = s;
     a.dlev[(long long)s * kMaxSources + t] = 0;
@@ -1919,12 +1923,13 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
     MbfsState* st = a.st;
     unsigned cnt = 0;
     unsigned long long E = 0;
+    // the sources' degrees were loaded by their own threads above: this
+    // loop reads LDS only
     for (int i = 0; i < a.nsrc; ++i) {
       if (!first[i]) continue;
-      const int s = a.sources[i];
-      a.q[0][cnt] = s;
+      a.q[0][cnt] = src[i];
       a.qe[0][cnt] = (uint32_t)E;
-      E += (unsigned long long)(a.off[s + 1] - a.off[s]);
+      E += deg[i];
       ++cnt;
     }
     for (int j = 0; j < 3; ++j) {
