@@ -73,7 +73,7 @@ typedef struct {
     int32_t bu_to_td_div;   /* default 20 */
     int32_t grid_blocks;    /* workgroups per level launch; 0 = auto (occupancy x CUs) */
     int32_t bu_serial_depth;/* row entries a lane tests alone (bottom-up) before its
-                               wavefront takes over the row; default 32 */
+                               wavefront takes over the row; default 128 */
     int32_t chunk_levels;   /* levels per captured HIP graph launch; default 16.  After
                                the first run on a graph, a run's first chunk is that
                                graph's previous launch count plus one (at most 1024) */

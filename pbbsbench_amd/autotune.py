@@ -41,6 +41,7 @@ SINGLE_GRID = {
     "td_to_bu_div": [5, 10, 20, 40, 80],
     "bu_to_td_div": [5, 10, 20, 40, 80, 160],
     "chunk_levels": [4, 16, 64],
+    "bu_serial_depth": [32, 64, 128, 256],
 }
 BATCH_GRID = {
     "batch_pull_div": [3, 4, 6, 10, 14, 20],

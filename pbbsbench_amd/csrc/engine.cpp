@@ -211,7 +211,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->td_to_bu_div = 20;
   t->bu_to_td_div = 20;
   t->grid_blocks = 0;
-  t->bu_serial_depth = 32;
+  t->bu_serial_depth = 128;
   t->chunk_levels = 16;
   t->use_graphs = 1;
   t->batch_pull_div = 10;

@@ -92,6 +92,7 @@ def test_permuted_rmat(ctx):
     dict(td_to_bu_div=1_000_000, bu_to_td_div=1),  # bottom-up once entered
     dict(bu_serial_depth=0),
     dict(bu_serial_depth=16),
+    dict(bu_serial_depth=4096),
     dict(grid_blocks=1),
     dict(grid_blocks=7),
     dict(use_graphs=0),
