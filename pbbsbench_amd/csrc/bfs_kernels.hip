@@ -1751,7 +1751,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
     __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   {
-    // F_{k+2}'s masks held F_{k-2}: level k reads F_{k-1} and F_k only.
+    // F_{k+2}'s masks held F_{k-2}: level k reads F_k's masks and writes F_{k+1}'s only.
     const gptr<unsigned long long> z = glob(A->fr[(k + 2) & 3]);
     const gptr<unsigned long long> zb = glob(A->fbm[(k + 2) & 3]);
     const long long n = A->n, nw = A->nwords;
@@ -1872,18 +1872,17 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void mbfs_clear_kernel(MbfsArgs<OffT> a) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nth = (long long)gridDim.x * blockDim.x;
+  // F_0 (seeded next) and F_1 (written by level 0) start empty.  F_2 and F_3
+  // are cleared by levels 0 and 1, as every level k clears F_{k+2} before
+  // level k+1 writes it, and no level reads F_{k-1}.
   for (long long v = tid; v < a.n; v += nth) {
     a.vis[v] = 0;
     a.fr[0][v] = 0;
     a.fr[1][v] = 0;
-    a.fr[2][v] = 0;
-    a.fr[3][v] = 0;
   }
   for (long long w = tid; w < a.nwords; w += nth) {
     a.fbm[0][w] = 0;
     a.fbm[1][w] = 0;
-    a.fbm[2][w] = 0;
-    a.fbm[3][w] = 0;
   }
   const long long total = (long long)a.nsrc * a.n;   // depth: mbfs_depth_kernel
   if ((reinterpret_cast<uintptr_t>(a.parent) & 15) == 0 && (total & 3) == 0) {
