@@ -3,27 +3,41 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload rmat24]
 
-A step is one single-source BFS over the whole graph (init + every level),
-with the CSR already resident in HBM and the parent/depth outputs written to
-HBM.  The default workload is the RMAT graph the target is quoted on:
-2^24 vertices, 2^28 directed draws with (a,b,c,d) = (.57,.19,.19,.05),
-self-loops and duplicates removed, symmetrised, int64 offsets, source = a
-seeded random non-isolated vertex.
+The default run measures the three parts of the metric and prints ONE JSON
+line:
 
-Prints one JSON line.  `value` is GTEPS = m_r / t with m_r the directed CSR
-entries of the reached vertices and t the wall-clock time per step over the
-timed steps.  `roofline` prices one BFS launch sequence against HBM:
-algorithmic bytes B = S_o*(n+1) + 4*m_r + 8*n over the hipEvent time of the
-sequence, measured on the context's stream over the timed region.
+* single-source BFS (the top-level line).  A step is one BFS over the whole
+  graph (init + every level), with the CSR already resident in HBM and the
+  parent/depth outputs written to HBM.  The default workload is the RMAT graph
+  the target is quoted on: 2^24 vertices, 2^28 directed draws with
+  (a,b,c,d) = (.57,.19,.19,.05), self-loops and duplicates removed,
+  symmetrised, int64 offsets, source = a seeded random non-isolated vertex.
+  `value` is GTEPS = m_r / t with m_r the directed CSR entries of the reached
+  vertices and t the wall-clock time per step over the K timed steps.
+  `roofline` prices one BFS launch sequence against HBM: algorithmic bytes
+  B = S_o*(n+1) + 4*m_r + 8*n over the hipEvent time of the sequence,
+  measured on the context's stream over the timed region.
+* `batched`: 64 seeded sources in one traversal on RMAT 2^22 against 64
+  sequential single-source runs of the same sources (ratio by wall time and
+  by device time), bit-exact against those runs and a sample of the oracle.
+* `serving`: 8 resident RMAT graphs of 2^16..2^22 vertices, a burst and then
+  Poisson arrivals; p50/p99 latency (arrival -> parents in host memory),
+  throughput and mean batch; a seeded sample compared with the oracle.
+
 `cpu_baseline` times the oracle's C restatement of deterministicBFS on the
-host cores (the reference's parlaylib build cannot be compiled here), and the
-same run checks the GPU output bit for bit.
+host cores with the protocol of common/time_loop.h:4-18 (>= 1 s of untimed
+warm-up, then timed rounds; min and median reported), since the reference's
+parlaylib build cannot be compiled here.  The same run checks the GPU output
+bit for bit.  Per-graph search data (row-top records, share bounds, degree
+ranks) are built before the timed region; their time and a cold first call
+are reported in `config`.
 """
 import argparse
 import json
 import os
 import sys
 import time
+import zlib
 
 import numpy as np
 
@@ -33,6 +47,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("GTEPS + % HBM roofline (1 GPU); batched-64-source vs sequential GTEPS; "
           "p50/p99 latency")
 HBM_PEAK_GBS = 8000.0
+TUNING_TABLE = os.path.join(ROOT, "pbbsbench_amd", "tuning_mi355x.json")
 
 # GPU builders (ctx, wide) -> DeviceGraph; the same graphs as the host builders,
 # bit for bit (tests/test_gpu_graphgen.py), built in HBM in milliseconds.
@@ -44,7 +59,7 @@ GPU_BUILDERS = {
 }
 
 WORKLOADS = {
-    # name: (builder, wide offsets, description)
+    # name: (host builder, wide offsets, description)
     "rmat24": (lambda: __import__("pbbsbench_amd").rmat(24, 2 ** 28, 0.57, 0.19, 0.19, seed=1),
                True, "RMAT 2^24 vertices, 2^28 draws, a=.57 b=.19 c=.19, symmetrised, int64 offsets"),
     "rmat16": (lambda: __import__("pbbsbench_amd").rmat(16, 2 ** 19, 0.5, 0.1, 0.1, seed=1),
@@ -57,6 +72,7 @@ WORKLOADS = {
                True, "RMAT 2^22 vertices, 2^26 draws, a=.57 b=.19 c=.19, symmetrised, int64 offsets"),
 }
 SOURCE_ZERO = {"rmat16", "randlocal10m", "torus216"}
+SERVE_GRAPHS = [(16, 1), (17, 1), (18, 1), (19, 1), (20, 1), (21, 1), (22, 1), (22, 2)]
 
 
 def pick_source(g, name, seed):
@@ -83,42 +99,78 @@ def cpu_info():
     return model
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
-    ap.add_argument("--seed", type=int, default=1, help="source-selection seed")
-    ap.add_argument("--cpu-seconds", type=float, default=10.0,
-                    help="budget of the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--td-to-bu-div", type=int, default=None)
-    ap.add_argument("--bu-to-td-div", type=int, default=None)
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "ingest"],
-                    help="single: one-source BFS (default); batch: 64 sources in one traversal "
-                         "vs 64 sequential runs on RMAT 2^22; serve: Poisson query stream over "
-                         "8 resident RMAT graphs; ingest: graph file (text / binary) into HBM")
-    ap.add_argument("--qps", type=float, default=0.0,
-                    help="serve: Poisson arrival rate (0 = half the measured burst peak)")
-    ap.add_argument("--queries", type=int, default=2000, help="serve: queries in the stream")
-    ap.add_argument("--spread", type=int, default=16,
-                    help="single: also time this many other seeded non-isolated sources once "
-                         "the timed region is over and report their spread (0 = skip)")
-    ap.add_argument("--host-gen", action="store_true",
-                    help="build the graphs with the host generators (default: on the GPU)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
-              "multi-GPU path: batched and serving work shard by replicating the graph per GPU, "
-              "which adds no figure beyond the one-GPU value.", file=sys.stderr)
-        sys.exit(2)
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
 
-    if args.mode == "batch":
-        return bench_batch(args)
-    if args.mode == "serve":
-        return bench_serve(args)
-    if args.mode == "ingest":
-        return bench_ingest(args)
+
+def apply_tuning(ctx, dg, use_table):
+    """Applies the committed tuning table's entry for dg's class (if any) and
+    returns what config records about it."""
+    from pbbsbench_amd import autotune
+    if not use_table or not os.path.exists(TUNING_TABLE):
+        return {"table": None, "class": autotune.graph_class(dg.info), "applied": False}
+    key = autotune.apply(ctx, dg, autotune.load_table(TUNING_TABLE))
+    t = ctx.tuning
+    return {"table": os.path.relpath(TUNING_TABLE, ROOT), "class": autotune.graph_class(dg.info),
+            "applied": key is not None,
+            "params": {k: int(getattr(t, k)) for k, _ in t._fields_}}
+
+
+def cpu_baseline(g, sources, budget_s, label):
+    """The oracle's deterministicBFS restatement timed with the time_loop
+    protocol (common/time_loop.h:4-18): untimed runs for at least one second,
+    then up to 5 timed rounds while the budget lasts.  `sources` are cycled
+    (one source for the single-source line).  Returns (record, outputs of the
+    first source)."""
+    import oracle  # the checker and the CPU baseline; never the measured product
+    threads = host_threads()
+    deg = g.degrees()
+    t0 = time.perf_counter()
+    warm = 0
+    first = None
+    while time.perf_counter() - t0 < 1.0:
+        p, _ = oracle.det_bfs(g.offsets, g.adj, int(sources[warm % len(sources)]), threads=threads)
+        if warm == 0:
+            first = p
+        warm += 1
+    times, edges = [], []
+    t1 = time.perf_counter()
+    while len(times) < 5 and (not times or time.perf_counter() - t1 + times[-1] < budget_s):
+        s = int(sources[len(times) % len(sources)])
+        t2 = time.perf_counter()
+        p, _ = oracle.det_bfs(g.offsets, g.adj, s, threads=threads)
+        times.append(time.perf_counter() - t2)
+        edges.append(int(deg[p != -1].sum()))
+    tmed = float(np.median(times))
+    rate = float(np.median([e / t for e, t in zip(edges, times)]))
+    rec = {
+        "value": round(rate / 1e9, 3),
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "min_s": round(float(min(times)), 4),
+        "median_s": round(tmed, 4),
+        "sample": (f"{label}: {warm} untimed warm-up runs (>= 1 s), then {len(times)} timed "
+                   f"rounds (min {min(times):.3f} s, median {tmed:.3f} s) of the oracle's C "
+                   f"restatement of deterministicBFS (parlaylib unavailable), {threads} host "
+                   f"threads; CPU: {cpu_info()}"),
+    }
+    return rec, first
+
+
+def traffic_from_profile(name, **match):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        tr = json.load(f)
+    if all(tr.get(k) == v for k, v in match.items()):
+        return tr.get("bytes_per_bfs", tr.get("bytes_per_batch"))
+    return None
+
+
+# ---------------------------------------------------------------- single source
+def run_single(args):
     import pbbsbench_amd as pb
 
     build, wide, desc = WORKLOADS[args.workload]
@@ -128,24 +180,37 @@ def main():
         tune["td_to_bu_div"] = args.td_to_bu_div
     if args.bu_to_td_div is not None:
         tune["bu_to_td_div"] = args.bu_to_td_div
-    if tune:
-        ctx.set_tuning(**tune)
     t0 = time.time()
-    if args.workload in GPU_BUILDERS and not args.host_gen:
+    on_gpu = args.workload in GPU_BUILDERS and not args.host_gen
+    if on_gpu:
         dg = GPU_BUILDERS[args.workload](ctx, wide)   # built in HBM
-        gen_where = "gpu"
         gen_s = time.time() - t0
         g = dg.download()   # host copy for source choice, CPU baseline and check
         upload_s = 0.0
     else:
         g = build()
-        gen_where = "host"
         gen_s = time.time() - t0
         t0 = time.time()
         dg = ctx.upload(g, wide_offsets=wide)
         upload_s = time.time() - t0
+    tuning = apply_tuning(ctx, dg, not tune and not args.no_tuning_table)
+    if tune:
+        ctx.set_tuning(**tune)
+        tuning = {"table": None, "applied": False, "params": dict(tune)}
     src = pick_source(g, args.workload, args.seed)
     deg = g.degrees()
+
+    # A cold call: the first BFS on a graph builds its search data (row-top
+    # records, share bounds) and captures the launch graphs.  Then a second
+    # copy of the graph times pbbs_graph_prepare alone.
+    t0 = time.perf_counter()
+    dg.bfs_device(src)
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    prep_ms = None
+    if on_gpu:
+        dg2 = GPU_BUILDERS[args.workload](ctx, wide)
+        prep_ms = dg2.prepare(single=True, batch=False)
+        dg2.free()
 
     for _ in range(args.warmup):
         dg.bfs_device(src)
@@ -164,9 +229,8 @@ def main():
     gteps = m_r / (ms_per_step / 1e3) / 1e9
 
     # Device time depends on the source: a source whose big level is reached
-    # through a frontier of a few low-id hubs costs up to 3x the bench source
-    # (a long top-down claim level, or bottom-up rows scanned down to their
-    # smallest ids).  Other seeded sources are timed after the timed region.
+    # through a frontier of a few low-id hubs costs more than the bench source.
+    # Other seeded sources are timed after the timed region.
     spread = None
     if args.spread > 0:
         rng = np.random.default_rng(args.seed + 1000)
@@ -178,6 +242,7 @@ def main():
             sms.append(float(np.median([r["device_ms"] for r in rs])))
             smr.append(rs[-1]["edges_reached"])
         sms_a = np.array(sms)
+        So_b = [So * (n + 1) + 4 * e + 8 * n for e in smr]
         spread = {
             "sources": int(args.spread), "seed": args.seed + 1000,
             "event_ms_min": round(float(sms_a.min()), 4),
@@ -185,6 +250,8 @@ def main():
             "event_ms_mean": round(float(sms_a.mean()), 4),
             "event_ms_max": round(float(sms_a.max()), 4),
             "gteps_device_aggregate": round(float(sum(smr)) / (float(sms_a.sum()) / 1e3) / 1e9, 2),
+            "roofline_frac_aggregate": round(float(sum(So_b)) / (float(sms_a.sum()) / 1e3) / 1e9
+                                             / HBM_PEAK_GBS, 4),
         }
 
     # outputs to host once (D2H reported, not timed) and bit-exact check
@@ -211,7 +278,15 @@ def main():
             "event_ms_per_bfs": round(event_ms, 4),
             "gteps_graph500": round(gteps / 2, 2),
             "d2h_ms": round(st_host["copy_ms"], 3),
-            "gen_s": round(gen_s, 3), "gen": gen_where, "upload_s": round(upload_s, 2),
+            "cold_first_bfs_ms": round(cold_ms, 3),
+            "preprocess_ms": None if prep_ms is None else round(prep_ms, 3),
+            "preprocess_note": ("per-graph search data built once and kept (row-top records, "
+                                "bottom-up share bounds; pbbs_graph_prepare), outside the timed "
+                                "region; cold_first_bfs_ms is the first BFS on a fresh graph, "
+                                "which builds them and captures the launch graphs"),
+            "gen_s": round(gen_s, 3), "gen": "gpu" if on_gpu else "host",
+            "upload_s": round(upload_s, 2),
+            "tuning": tuning,
             "parallelism": "single GPU",
             "source_spread": spread,
         },
@@ -222,52 +297,30 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(B / (event_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic_from_profile(args.workload, n=n, m=g.m, source=src),
             "algorithmic_bytes": int(B),
         },
     }
-    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(traffic_file):
-        with open(traffic_file) as f:
-            tr = json.load(f)
-        if tr.get("n") == n and tr.get("m") == g.m and tr.get("source") == src:
-            result["roofline"]["traffic"] = tr["bytes_per_bfs"]
-
+    exact = True
     if args.cpu_seconds > 0:
-        import oracle  # the checker and the CPU baseline; never the measured product
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        times = []
-        tstart = time.time()
-        ref = None
-        while not times or (time.time() - tstart < args.cpu_seconds and len(times) < 5):
-            t1 = time.perf_counter()
-            ref, _ = oracle.det_bfs(g.offsets, g.adj, src, threads=threads)
-            times.append(time.perf_counter() - t1)
-        exact = bool(np.array_equal(ref, parent)) and bool(
-            np.array_equal(oracle.level_number(ref), depth))
-        tmed = float(np.median(times))
-        result["cpu_baseline"] = {
-            "value": round(m_r / tmed / 1e9, 3),
-            "unit": "GTEPS",
-            "cores": threads,
-            "kind": "port",
-            "sample": (f"{len(times)} full BFS runs from the bench source on the same graph "
-                       f"(median {tmed:.3f} s), oracle's C restatement of deterministicBFS "
-                       f"(parlaylib unavailable); CPU: {cpu_info()}"),
-        }
+        rec, ref = cpu_baseline(g, [src], args.cpu_seconds,
+                                f"full BFS runs from the bench source on {args.workload}")
+        exact = bool(np.array_equal(ref, parent))
+        import oracle
+        exact = exact and bool(np.array_equal(oracle.level_number(ref), depth))
+        result["cpu_baseline"] = rec
         result["parity"] = "bit-exact parent and depth vs oracle" if exact else "MISMATCH"
-        if not exact:
-            print(json.dumps(result))
-            sys.exit(1)
     dg.free()
     ctx.close()
-    print(json.dumps(result))
+    return result, exact
 
 
-def bench_batch(args):
+# ---------------------------------------------------------------------- batched
+def run_batch(args, nested):
     """64 sources in one batched traversal vs 64 sequential single-source runs
     on RMAT 2^22 (2^26 draws, a=.57 b=.19 c=.19), uint32 offsets.  A step is
     one batched BFS of all 64 sources with outputs [64][n] written to HBM."""
+    import oracle
     import pbbsbench_amd as pb
     ctx = pb.Context(0)
     t0 = time.time()
@@ -279,12 +332,18 @@ def bench_batch(args):
         dg = GPU_BUILDERS["rmat22"](ctx, False)
         gen_s = time.time() - t0
         g = dg.download()
+    tuning = apply_tuning(ctx, dg, not args.no_tuning_table)
+    prep_ms = dg.prepare(single=True, batch=True)
     deg = g.degrees()
     rng = np.random.default_rng(args.seed)
     src = rng.choice(np.flatnonzero(deg > 0), 64, replace=False).astype(np.int32)
+    t1 = time.perf_counter()
+    dg.bfs_batch_device(src)
+    first_ms = (time.perf_counter() - t1) * 1e3
     for _ in range(max(args.warmup, 1)):
         dg.bfs_batch_device(src)
-        dg.bfs_device(int(src[0]))
+        for s in src:
+            dg.bfs_device(int(s))
     # sequential: 64 single-source runs, outputs in HBM
     seq_ms, seq_edges = 0.0, 0
     t0 = time.perf_counter()
@@ -301,105 +360,89 @@ def bench_batch(args):
         dev_ms.append(st["device_ms"])
     wall = (time.perf_counter() - t0) / args.steps * 1e3
     pairs_edges = st["edges_reached"]
-    assert pairs_edges == seq_edges, (pairs_edges, seq_edges)
-    # parity: every batched row equals its sequential run, bit for bit
+    exact = pairs_edges == seq_edges
+    # parity: every batched row equals its sequential run, bit for bit, and a
+    # seeded sample of rows equals the oracle
     P, D, _ = dg.bfs_batch(src)
-    exact = True
     for i, s in enumerate(src):
         p, d, _ = dg.bfs(int(s))
         exact &= bool(np.array_equal(P[i], p)) and bool(np.array_equal(D[i], d))
+    sample = rng.choice(64, 4, replace=False)
+    for i in sample:
+        ref, _ = oracle.det_bfs(g.offsets, g.adj, int(src[i]), threads=host_threads())
+        exact &= bool(np.array_equal(ref, P[i])) and bool(
+            np.array_equal(oracle.level_number(ref), D[i]))
     reached_any = (P != -1).any(axis=0)
     m_union = int(deg[reached_any].sum())
     n = g.n
     B = 4 * (n + 1) + 4 * m_union + 8 * 64 * n
     ev = float(np.mean(dev_ms))
-    result = {
-        "metric": "batched-64-source aggregate GTEPS (vs 64 sequential runs)",
-        "value": round(pairs_edges / (wall / 1e3) / 1e9, 2),
-        "unit": "GTEPS",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(wall, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic (seeded generator in libpbbs_bfs)",
-        "config": {
-            "workload": "rmat22x64",
-            "description": "RMAT 2^22 vertices, 2^26 draws, a=.57 b=.19 c=.19, symmetrised, "
-                           "uint32 offsets; 64 seeded non-isolated sources",
-            "n": n, "m": g.m, "sources_seed": args.seed,
-            "pairs_reached": st["reached"], "sum_m_r": pairs_edges,
-            "levels": st["levels"], "push_levels": st["td_levels"],
-            "pull_levels": st["bu_levels"],
-            "event_ms_per_batch": round(ev, 3),
-            "sequential_event_ms_64": round(seq_ms, 3),
-            "sequential_wall_ms_64": round(seq_wall, 3),
-            "sequential_gteps": round(seq_edges / (seq_wall / 1e3) / 1e9, 2),
-            "batch_vs_sequential": round(seq_wall / wall, 3),
-            "gen_s": round(gen_s, 3), "gen": "host" if args.host_gen else "gpu",
-            "parallelism": "single GPU",
-        },
+    res = {
+        "workload": "rmat22x64",
+        "description": "RMAT 2^22 vertices, 2^26 draws, a=.57 b=.19 c=.19, symmetrised, "
+                       "uint32 offsets; 64 seeded non-isolated sources in one traversal",
+        "n": n, "m": g.m, "sources_seed": args.seed, "steps": args.steps,
+        "value_gteps": round(pairs_edges / (wall / 1e3) / 1e9, 2),
+        "ms_per_batch": round(wall, 3),
+        "event_ms_per_batch": round(ev, 3),
+        "pairs_reached": st["reached"], "sum_m_r": pairs_edges,
+        "levels": st["levels"], "push_levels": st["td_levels"], "pull_levels": st["bu_levels"],
+        "sequential_wall_ms_64": round(seq_wall, 3),
+        "sequential_event_ms_64": round(seq_ms, 3),
+        "sequential_gteps": round(seq_edges / (seq_wall / 1e3) / 1e9, 2),
+        "batch_vs_sequential": round(seq_wall / wall, 3),
+        "batch_vs_sequential_device": round(seq_ms / ev, 3),
+        "preprocess_ms": round(prep_ms, 3),
+        "first_batch_ms": round(first_ms, 3),
+        "gen_s": round(gen_s, 3), "gen": "host" if args.host_gen else "gpu",
+        "tuning": tuning,
         "roofline": {
             "bound": "hbm",
-            "kernel": "one batched BFS launch sequence (mbfs clear/seed + mbfs_level_kernel per level)",
+            "kernel": "one batched BFS launch sequence (mbfs clear/seed, mbfs_level_kernel per "
+                      "level, mbfs_depth_kernel)",
             "achieved": round(B / (ev / 1e3) / 1e9, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(B / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-            "traffic": None, "algorithmic_bytes": int(B),
+            "traffic": traffic_from_profile("rmat22x64", n=n, m=g.m, sources_seed=args.seed),
+            "algorithmic_bytes": int(B),
         },
-        "parity": "bit-exact vs sequential runs" if exact else "MISMATCH",
+        "parity": ("bit-exact vs the 64 sequential runs and vs the oracle on 4 seeded rows"
+                   if exact else "MISMATCH"),
     }
-    if args.cpu_seconds > 0:
-        import oracle  # the checker and the CPU baseline; never the measured product
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        t1 = time.time()
-        done, edges, secs = 0, 0, 0.0
-        for s in src:
-            t2 = time.perf_counter()
-            ref, _ = oracle.det_bfs(g.offsets, g.adj, int(s), threads=threads)
-            secs += time.perf_counter() - t2
-            edges += int(deg[ref != -1].sum())
-            exact &= bool(np.array_equal(ref, P[done]))
-            done += 1
-            if time.time() - t1 > args.cpu_seconds:
-                break
-        result["cpu_baseline"] = {
-            "value": round(edges / secs / 1e9, 3), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle det_bfs from the first {done} of the 64 sources, one after another "
-                      f"({secs:.2f} s); CPU: {cpu_info()}",
-        }
-        result["parity"] = ("bit-exact vs sequential runs and vs oracle on the CPU sample"
-                            if exact else "MISMATCH")
+    if not nested and args.cpu_seconds > 0:
+        rec, _ = cpu_baseline(g, src, args.cpu_seconds,
+                              "single-source BFS runs from the batch's sources, one after another")
+        res["cpu_baseline"] = rec
     dg.free()
     ctx.close()
-    print(json.dumps(result))
-    if not exact:
-        sys.exit(1)
+    return res, exact
 
 
-SERVE_GRAPHS = [(16, 1), (17, 1), (18, 1), (19, 1), (20, 1), (21, 1), (22, 1), (22, 2)]
-
-
-def bench_serve(args):
+# ---------------------------------------------------------------------- serving
+def run_serve(args, nested):
     """Query stream over 8 resident RMAT graphs (scales 16..22 plus a second
     2^22 graph; 16 draws per vertex, a=.57 b=.19 c=.19).  Each query picks a
     graph and a non-isolated source at random and asks for parents in a host
     buffer.  First a burst of queries submitted at once gives the peak
     throughput; then Poisson arrivals at --qps (default: half that peak) give
     the latency percentiles, arrival -> parents in the caller's buffer.  A
-    seeded tenth of the Poisson queries is re-run alone afterwards and must
-    match bit for bit.  Output buffers come from per-graph pools allocated and
-    touched before timing, as a serving client reuses its buffers."""
+    seeded sample of the Poisson queries is compared with the oracle.  Output
+    buffers come from per-graph pools allocated and touched before timing, as
+    a serving client reuses its buffers."""
     import threading
-    import zlib
 
+    import oracle
     import pbbsbench_amd as pb
     rng = np.random.default_rng(args.seed)
     graphs, nz = [], []
     t0 = time.time()
-    for scale, seed in SERVE_GRAPHS:
-        g = pb.rmat(scale, 16 * 2 ** scale, 0.57, 0.19, 0.19, seed=seed)
-        graphs.append(g)
-        nz.append(np.flatnonzero(g.degrees() > 0))
+    with pb.Context(0) as gctx:   # the same graphs as pb.rmat(...), built on the GPU
+        for scale, seed in SERVE_GRAPHS:
+            dg = gctx.gen_rmat(scale, 16 * 2 ** scale, 0.57, 0.19, 0.19, seed=seed)
+            g = dg.download()
+            dg.free()
+            graphs.append(g)
+            nz.append(np.flatnonzero(g.degrees() > 0))
     gen_s = time.time() - t0
     G = len(graphs)
 
@@ -431,8 +474,9 @@ def bench_serve(args):
     bufs = [take(gi) for gi in bsel]
     t0 = time.perf_counter()
     tks = [srv.submit(ids[bsel[q]], bsrc[q], out=(bufs[q], None)) for q in range(burst)]
-    srv.drain(tks)
+    got = srv.drain(tks)
     peak = burst / (time.perf_counter() - t0)
+    burst_batch = float(np.mean([got[t][0]["batch"] for t in tks]))
     for q in range(burst):
         give(bsel[q], bufs[q])
     qps = args.qps if args.qps > 0 else 0.5 * peak
@@ -440,7 +484,7 @@ def bench_serve(args):
     # Poisson arrivals
     gsel, srcs = draw(Q)
     gaps = rng.exponential(1.0 / qps, Q)
-    check = set(rng.choice(Q, max(1, Q // 10), replace=False).tolist())
+    check = set(rng.choice(Q, min(Q, args.serve_check), replace=False).tolist())
     for gi in range(G):   # buffers for the expected number in flight, touched now
         need = int(np.ceil(qps * 0.05)) + 64
         while len(pools[gi]) < need:
@@ -473,54 +517,38 @@ def bench_serve(args):
             give(gsel[q], buf)
     th.join()
     t_total = time.perf_counter() - t_start
+    load = srv.load()
     srv.close()
     lat = np.array([results[q][0]["latency_ms"] for q in range(Q)])
     qms = np.array([results[q][0]["queue_ms"] for q in range(Q)])
     batch = np.array([results[q][0]["batch"] for q in range(Q)])
-    ok = all(results[q][0]["status"] == 0 for q in range(Q))
-    # solo re-runs of the checked queries
-    ctx = pb.Context(0)
-    exact = ok
-    for gi in range(G):
-        qs = [q for q in check if gsel[q] == gi]
-        if not qs:
-            continue
-        dg = ctx.upload(graphs[gi])
-        for q in qs:
-            p, _, _ = dg.bfs(srcs[q], depth=False)
-            exact &= zlib.crc32(p.tobytes()) == results[q][1]
-        dg.free()
-    ctx.close()
-    result = {
-        "metric": "BFS serving: p50/p99 latency (arrival -> parents in host memory)",
-        "value": round(float(np.percentile(lat, 99)), 3),
-        "unit": "ms (p99)",
-        "n_gpus": 1, "steps": Q, "warmup": G,
-        "ms_per_step": round(t_total / Q * 1e3, 4),
-        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "int32", "data": "synthetic (seeded generator in libpbbs_bfs)",
-        "config": {
-            "workload": "serve8",
-            "graphs": [f"rmat{s} seed {sd} n={g.n} m={g.m}"
-                       for (s, sd), g in zip(SERVE_GRAPHS, graphs)],
-            "peak_qps_burst": round(peak, 1), "burst_queries": burst,
-            "qps_offered": round(qps, 1), "queries": Q,
-            "throughput_qps": round(Q / t_total, 1),
-            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p90_ms": round(float(np.percentile(lat, 90)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
-            "queue_p50_ms": round(float(np.percentile(qms, 50)), 3),
-            "mean_batch": round(float(batch.mean()), 2),
-            "checked_solo": len(check), "gen_s": round(gen_s, 1),
-            "parallelism": "single GPU",
-        },
-        "parity": "bit-exact vs solo runs (checked sample)" if exact else "MISMATCH",
+    exact = all(results[q][0]["status"] == 0 for q in range(Q))
+    # the checked sample against the oracle
+    for q in sorted(check):
+        g = graphs[gsel[q]]
+        ref, _ = oracle.det_bfs(g.offsets, g.adj, srcs[q], threads=host_threads())
+        exact &= zlib.crc32(ref.tobytes()) == results[q][1]
+    res = {
+        "workload": "serve8",
+        "graphs": [f"rmat{s} seed {sd} n={g.n} m={g.m}"
+                   for (s, sd), g in zip(SERVE_GRAPHS, graphs)],
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p90_ms": round(float(np.percentile(lat, 90)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "throughput_qps": round(Q / t_total, 1),
+        "qps_offered": round(qps, 1), "queries": Q,
+        "peak_qps_burst": round(peak, 1), "burst_queries": burst,
+        "burst_mean_batch": round(burst_batch, 2),
+        "mean_batch": round(float(batch.mean()), 2),
+        "queue_p50_ms": round(float(np.percentile(qms, 50)), 3),
+        "hbm_charged_bytes": load["hbm_in_use"],
+        "checked_vs_oracle": len(check), "gen_s": round(gen_s, 1),
+        "parity": ("bit-exact vs the oracle on the checked sample" if exact else "MISMATCH"),
     }
-    print(json.dumps(result))
-    if not exact:
-        sys.exit(1)
+    return res, exact
 
 
+# ----------------------------------------------------------------------- ingest
 def bench_ingest(args):
     """Graph file -> HBM.  The workload's graph is built on the GPU, downloaded
     and written as an AdjacencyGraph text file and a binary CSR file in a
@@ -532,7 +560,6 @@ def bench_ingest(args):
     import ctypes
     import shutil
     import tempfile
-    import zlib
 
     import pbbsbench_amd as pb
     from pbbsbench_amd import _lib
@@ -596,11 +623,90 @@ def bench_ingest(args):
         "binary_GBps": round(sizes["binary"] / res["binary"] / 1e9, 2),
         "host_text_parse_s": round(float(np.median(host)), 3),
         "text_write_s": round(write_s, 2),
-        "host_threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+        "host_threads": host_threads(),
         "parity": "bit-exact (crc32 of the downloaded CSR)",
     }
     ctx.close()
     print(json.dumps(result))
+
+
+def line(base, **over):
+    """A stand-alone JSON line for the batch / serve modes."""
+    out = {"n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+           "data": "synthetic (seeded generator in libpbbs_bfs)"}
+    out.update(base)
+    out.update(over)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="rmat24", choices=sorted(WORKLOADS))
+    ap.add_argument("--seed", type=int, default=1, help="source-selection seed")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="budget of the CPU-baseline timed rounds after its 1 s warm-up (0 = skip)")
+    ap.add_argument("--td-to-bu-div", type=int, default=None)
+    ap.add_argument("--bu-to-td-div", type=int, default=None)
+    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "ingest"],
+                    help="single: one-source BFS with the batched and serving parts nested in "
+                         "its line (default); batch: 64 sources in one traversal vs 64 "
+                         "sequential runs on RMAT 2^22 alone; serve: Poisson query stream over "
+                         "8 resident RMAT graphs alone; ingest: graph file (text / binary) into HBM")
+    ap.add_argument("--no-batched", action="store_true", help="single: skip the batched part")
+    ap.add_argument("--no-serving", action="store_true", help="single: skip the serving part")
+    ap.add_argument("--no-tuning-table", action="store_true",
+                    help="run on the library defaults instead of the committed tuning table")
+    ap.add_argument("--qps", type=float, default=0.0,
+                    help="serve: Poisson arrival rate (0 = half the measured burst peak)")
+    ap.add_argument("--queries", type=int, default=2000, help="serve: queries in the stream")
+    ap.add_argument("--serve-check", type=int, default=48,
+                    help="serve: queries of the stream compared with the oracle")
+    ap.add_argument("--spread", type=int, default=16,
+                    help="single: also time this many other seeded non-isolated sources once "
+                         "the timed region is over and report their spread (0 = skip)")
+    ap.add_argument("--host-gen", action="store_true",
+                    help="build the graphs with the host generators (default: on the GPU)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
+              "multi-GPU path: batched and serving work shard by replicating the graph per GPU, "
+              "which adds no figure beyond the one-GPU value.", file=sys.stderr)
+        sys.exit(2)
+
+    if args.mode == "ingest":
+        return bench_ingest(args)
+    if args.mode == "batch":
+        res, exact = run_batch(args, nested=False)
+        print(json.dumps(line({
+            "metric": "batched-64-source aggregate GTEPS (vs 64 sequential runs)",
+            "value": res["value_gteps"], "unit": "GTEPS", "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_batch"],
+            "higher_is_better": True, "config": res,
+            "roofline": res["roofline"], "parity": res["parity"]},
+            **({"cpu_baseline": res.pop("cpu_baseline")} if "cpu_baseline" in res else {}))))
+        sys.exit(0 if exact else 1)
+    if args.mode == "serve":
+        res, exact = run_serve(args, nested=False)
+        print(json.dumps(line({
+            "metric": "BFS serving: p50/p99 latency (arrival -> parents in host memory)",
+            "value": res["p99_ms"], "unit": "ms (p99)", "steps": res["queries"],
+            "warmup": len(SERVE_GRAPHS), "ms_per_step": round(1e3 / res["throughput_qps"], 4),
+            "higher_is_better": False, "config": res, "parity": res["parity"]})))
+        sys.exit(0 if exact else 1)
+
+    result, exact = run_single(args)
+    if not args.no_batched:
+        result["batched"], ok = run_batch(args, nested=True)
+        exact &= ok
+    if not args.no_serving:
+        result["serving"], ok = run_serve(args, nested=True)
+        exact &= ok
+    print(json.dumps(result))
+    if not exact:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -85,6 +85,10 @@ typedef struct {
                                six largest neighbours, row end, degree), built in HBM on the
                                first BFS of a sorted symmetric graph (default); 0 = they read
                                offsets and row tops from the CSR (no extra HBM) */
+    int32_t queue_div;      /* the single-source frontier queues hold n / queue_div entries
+                               (default 4); a level whose next frontier does not fit stores
+                               the rest in the frontier bitmap only, and the level after it
+                               expands that bitmap (same outputs, bounded workspace) */
 } pbbs_tuning;
 
 int pbbs_tuning_get(const pbbs_ctx* ctx, pbbs_tuning* out);
@@ -121,6 +125,23 @@ int pbbs_graph_upload(pbbs_ctx* ctx, int64_t n, int64_t m, const void* offsets,
                       const int32_t* adj, uint32_t flags, pbbs_graph** out);
 int pbbs_graph_info_get(const pbbs_graph* g, pbbs_graph_info* out);
 int pbbs_graph_free(pbbs_graph* g);
+
+/* Builds a graph's search data now rather than on its first BFS, so that a
+ * first BFS costs what a later one does and the HBM it takes is known:
+ *   PBBS_PREPARE_SINGLE  the bottom-up share bounds for ctx's launch shape
+ *                        and, on a sorted symmetric graph with bu_row_tops
+ *                        on, the 32-byte row-top records (32 n bytes);
+ *   PBBS_PREPARE_BATCH   (symmetric graphs) degree ranks, the rank-id copy
+ *                        of the adjacency, the list of non-isolated rows and
+ *                        the pull slices for ctx's batched launch shape
+ *                        (12 n + 4 m bytes).
+ * Each piece is built once per graph and kept until pbbs_graph_free;
+ * pbbs_graph_info's device_bytes counts it.  *ms (may be NULL) receives the
+ * call's wall time.  No reference counterpart (the reference's BFS keeps no
+ * per-graph data between calls). */
+#define PBBS_PREPARE_SINGLE 0x1u
+#define PBBS_PREPARE_BATCH  0x2u
+int pbbs_graph_prepare(pbbs_ctx* ctx, pbbs_graph* g, uint32_t what, double* ms);
 
 /* ---- single-source BFS ------------------------------------------------- */
 #define PBBS_OUT_DEVICE   0x1u  /* parent/depth are device pointers (no D2H copy) */
@@ -180,7 +201,9 @@ typedef struct {
                               [-1, n)), -1 if none */
     int64_t bad;           /* vertices rejected */
     int64_t max_rule_bad;  /* vertices whose parent is not the largest neighbour one
-                              level up (-1 if unreached, itself for the source) */
+                              level up (-1 if unreached, itself for the source); -1 on a
+                              graph that is not symmetric, where a row does not list the
+                              in-neighbours the rule needs (the rule is not checked) */
     int64_t depth_bad;     /* vertices whose depth differs from their level */
     int64_t reached;       /* vertices with a level below n */
     int32_t max_level;
@@ -204,8 +227,10 @@ typedef struct pbbs_server pbbs_server;
 typedef struct {
     int32_t slots;        /* concurrent runs; 0 = 4 */
     int32_t max_batch;    /* queries merged per run, 1..64; 0 = 64; 1 = never merge */
-    int64_t hbm_budget;   /* workspace bytes of the runs in flight at once (a run is
-                             admitted, or its batch shrunk, to fit); 0 = 80% of free HBM */
+    int64_t hbm_budget;   /* HBM the server may hold: resident graphs with their search
+                             data, plus every slot's workspace buffers (kept between runs).
+                             A run is admitted, or its batch shrunk, to fit; idle slots'
+                             buffers are freed when needed.  0 = 80% of free HBM */
     int32_t reserved[4];
 } pbbs_server_config;
 
@@ -223,7 +248,9 @@ typedef struct {
 int pbbs_server_create(int device, const pbbs_server_config* cfg, pbbs_server** out);
 /* Finishes every submitted query, then releases everything. */
 int pbbs_server_destroy(pbbs_server* s);
-/* Uploads a CSR (as pbbs_graph_upload) and returns its id. */
+/* Uploads a CSR (as pbbs_graph_upload), builds its search data
+ * (pbbs_graph_prepare) and returns its id.  PBBS_ERR_LIMIT if the resident
+ * graphs plus one single-source run would exceed hbm_budget. */
 int pbbs_server_add_graph(pbbs_server* s, int64_t n, int64_t m, const void* offsets,
                           const int32_t* adj, uint32_t flags, int32_t* graph_id);
 /* Queues a BFS; parent/depth (host, n entries each, either may be NULL) must
@@ -233,6 +260,8 @@ int pbbs_server_submit(pbbs_server* s, int32_t graph_id, int32_t source, int32_t
 /* Takes up to max finished results; waits up to wait_ms (-1 = until one). */
 int pbbs_server_poll(pbbs_server* s, pbbs_server_result* out, int32_t max, int32_t* count,
                      int32_t wait_ms);
+/* Queries waiting, queries in runs, and the HBM bytes charged to the budget
+ * now (resident graphs with their search data + slots' workspace buffers). */
 int pbbs_server_load(pbbs_server* s, int64_t* queued, int64_t* running, int64_t* hbm_in_use);
 
 /* Per-level timing of one BFS, for tuning and measurement: every level kernel

@@ -22,6 +22,8 @@ PBBS_GRAPH_OFFSETS_I64 = 0x1
 PBBS_GRAPH_DEVICE_INPUT = 0x2
 PBBS_GRAPH_ASSUME_SYMMETRIC = 0x8
 PBBS_OUT_DEVICE = 0x1
+PBBS_PREPARE_SINGLE = 0x1
+PBBS_PREPARE_BATCH = 0x2
 
 
 class Tuning(C.Structure):
@@ -34,6 +36,7 @@ class Tuning(C.Structure):
         ("use_graphs", C.c_int32),
         ("batch_pull_div", C.c_int32),
         ("bu_row_tops", C.c_int32),
+        ("queue_div", C.c_int32),
     ]
 
 
@@ -145,6 +148,7 @@ SIGNATURES = [
      [_P, C.c_int64, C.c_int64, _P, _P, C.c_uint32, C.POINTER(_P)]),
     ("pbbs_graph_info_get", C.c_int, [_P, C.POINTER(GraphInfo)]),
     ("pbbs_graph_free", C.c_int, [_P]),
+    ("pbbs_graph_prepare", C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_double)]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_batch", C.c_int,
      [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),

@@ -321,6 +321,14 @@ class DeviceGraph:
         check(lib().pbbs_graph_download(self._h, C.byref(c)), "graph_download")
         return HostGraph._take(c)
 
+    def prepare(self, single=True, batch=True):
+        """Builds the graph's search data now (pbbs_graph_prepare) rather than
+        on its first BFS; returns the call's wall time in ms."""
+        what = (_lib.PBBS_PREPARE_SINGLE if single else 0) | (_lib.PBBS_PREPARE_BATCH if batch else 0)
+        ms = C.c_double()
+        check(lib().pbbs_graph_prepare(self.ctx._h, self._h, what, C.byref(ms)), "graph_prepare")
+        return ms.value
+
     @property
     def info(self):
         i = _lib.GraphInfo()

@@ -28,13 +28,16 @@ struct LevelRecord {
 // A frontier F_k is the concatenation of kSubQueues sub-queues; ctr[k % 3][s]
 // packs sub-queue s's vertex count (low 32 bits) and degree sum (high 32
 // bits), so one 64-bit atomicAdd reserves queue slots and edge offsets
-// together.  Degree-0 vertices are reached but not queued; d0[k % 3] counts
+// together.  A sub-queue region holds qcap entries (a quarter of n over all
+// sub-queues by default); appends past it are counted but not stored, and
+// qover[k % 3] tells level k to expand F_k from its bitmap instead.  Degree-0 vertices are reached but not queued; d0[k % 3] counts
 // those at depth k.  Workgroup 0 of level k's kernel does the bookkeeping
 // (stats, modes[k & 3], clearing the counters level k+1 will fill); every
 // workgroup derives the level's direction from the same inputs itself.
 struct BfsState {
   unsigned long long ctr[3][kSubQueues];
   uint32_t d0[3];
+  int32_t qover[3];                  // F_k's queue overflowed (entries dropped): expand its bitmap
   int32_t modes[4];
   int32_t finished;                  // set by the first kernel that sees F_k empty
   int32_t td_levels;
@@ -76,7 +79,7 @@ struct BfsArgs {
   int64_t n;
   int64_t m;
   int64_t nwords;              // ceil(n/64)
-  int64_t qcap;                // entries per sub-queue region (= n)
+  int64_t qcap;                // entries per sub-queue region
   int32_t td_to_bu_div;        // 0 = never bottom-up
   int32_t bu_to_td_div;
   int32_t bu_serial_depth;
@@ -93,6 +96,8 @@ constexpr int kMaxSources = 64;
 
 struct MbfsState {
   unsigned long long ctr[3][kSubQueues];  // union-frontier sub-queue counters, [k % 3]
+  int32_t qover[3];                       // never set: batched sub-queues hold n entries each
+  int32_t pad;
   unsigned long long unfinished[3];       // rows a pull left with sources to find, [(k+1) % 3]
   int32_t modes[4];
   int32_t finished;

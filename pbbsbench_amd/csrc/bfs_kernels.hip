@@ -237,6 +237,7 @@ struct LevelView {
   gptr<int32_t> q_next;                 // this workgroup's sub-queue region of F_{k+1}
   gptr<uint32_t> qe_next;
   gptr<unsigned long long> ctr_next;    // its counter
+  gptr<int32_t> qover_next;             // set when an append finds the sub-queue full
   gptr<uint32_t> d0_next;
   gptr<const unsigned long long> ctr_cur;  // F_k's counters (all sub-queues)
   gptr<const uint32_t> bu_bounds;       // weighted wavefront shares, or null for equal shares
@@ -272,14 +273,20 @@ struct Stage {
 };
 
 // Writes buffer entries [0, cnt) of wavefront w to the next frontier at
-// (qbase, ebase) and empties it.
+// (qbase, ebase) and empties it.  A sub-queue region holds qcap entries:
+// entries past it are dropped and the frontier is marked incomplete, so the
+// level that consumes it reads its bitmap instead.
 template <typename View, typename SB>
 __device__ __forceinline__ void stage_copy_out(const View& a, SB& sb, int w,
                                                unsigned cnt, unsigned qbase, unsigned ebase) {
-  for (unsigned i = lane_id(); i < cnt; i += kWave) {
+  const unsigned long long room =
+      (unsigned long long)a.qcap > qbase ? (unsigned long long)a.qcap - qbase : 0ull;
+  const unsigned lim = room < cnt ? (unsigned)room : cnt;
+  for (unsigned i = lane_id(); i < lim; i += kWave) {
     a.q_next[qbase + i] = sb.v[w][i];
     a.qe_next[qbase + i] = ebase + sb.eo[w][i];
   }
+  if (lim < cnt && lane_id() == 0) *a.qover_next = 1;
 }
 
 template <typename View, typename SB>
@@ -994,6 +1001,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   PBBS_STAMP(k, 0);
   const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
   const int mode = level_mode(*A, prev, cnt, E);
+  const bool qover = st->qover[c0] != 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->modes[k & 3] = mode;
     st->reached += cnt + st->d0[c0];
@@ -1009,6 +1017,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
 #pragma unroll
     for (int s = 0; s < kSubQueues; ++s) st->ctr[c2][s] = 0;
     st->d0[c2] = 0;
+    st->qover[c2] = 0;
     __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
   {
@@ -1043,6 +1052,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.q_next = glob(A->q[(k + 1) & 1] + sq * a.qcap);
   a.qe_next = glob(A->qe[(k + 1) & 1] + sq * a.qcap);
   a.ctr_next = glob(&st->ctr[c1][sq]);
+  a.qover_next = glob(&st->qover[c1]);
   a.d0_next = glob(&st->d0[c1]);
   a.ctr_cur = glob(st->ctr[c0]);
   // The first bottom-up level after top-down meets almost every vertex
@@ -1058,7 +1068,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.serial_depth = A->bu_serial_depth;
   Stage stg;
   if (mode == kTopDown) {
-    if (prev == kBottomUp)
+    if (prev == kBottomUp || qover)
       td_from_bitmap(a, sb, stg, bl);
     else
       top_down(a, sb, stg, E);
@@ -1105,6 +1115,7 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
     for (int j = 0; j < 3; ++j) {
       for (int q = 0; q < kSubQueues; ++q) st->ctr[j][q] = 0;
       st->d0[j] = 0;
+      st->qover[j] = 0;
     }
     if (deg) st->ctr[0][0] = (deg << 32) | 1ull;
     else st->d0[0] = 1;
@@ -1208,6 +1219,7 @@ struct MbfsView {
   gptr<int32_t> q_next;                     // this workgroup's sub-queue region
   gptr<uint32_t> qe_next;
   gptr<unsigned long long> ctr_next;
+  gptr<int32_t> qover_next;                 // never set: sub-queues hold n entries each
   gptr<const int32_t> pull_rows;            // first list entry of each wavefront's slice
   gptr<const int32_t> rows;                 // vertices of degree > 0, ascending
   long long nrows;
@@ -1783,6 +1795,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_level_kernel(const MbfsArgs<OffT>
   a.q_next = glob(A->q[(k + 1) & 1] + sq * a.qcap);
   a.qe_next = glob(A->qe[(k + 1) & 1] + sq * a.qcap);
   a.ctr_next = glob(&st->ctr[c1][sq]);
+  a.qover_next = glob(&st->qover[c1]);
   a.pull_rows = glob(A->pull_rows);
   a.rows = glob(A->rows);
   a.nrows = A->nrows;
@@ -1934,6 +1947,7 @@ __global__ __launch_bounds__(kBlock) void mbfs_seed_kernel(MbfsArgs<OffT> a, Mbf
     for (int j = 0; j < 3; ++j) {
       for (int q = 0; q < kSubQueues; ++q) st->ctr[j][q] = 0;
       st->unfinished[j] = 0;
+      st->qover[j] = 0;
     }
     st->ctr[0][0] = (E << 32) | cnt;
     for (int j = 0; j < 4; ++j) st->modes[j] = kTopDown;

@@ -9,6 +9,8 @@
 //      both n (BFSCheck.C:59-90); the max rule wants parent[v] = the largest
 //      u in N(v) with level[u] == level[v] - 1 (-1 if unreached); given
 //      depths must equal the levels.
+// The max rule reads N(v) as v's in-neighbours, which holds on symmetric
+// graphs only; on any other graph it is skipped and max_rule_bad is -1.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -28,6 +30,7 @@ int ctx_num_cus(const pbbs_ctx* c);
 const void* graph_dev_off(const pbbs_graph* g);
 const int32_t* graph_dev_adj(const pbbs_graph* g);
 bool graph_off64(const pbbs_graph* g);
+bool graph_symmetric(const pbbs_graph* g);
 int64_t graph_n(const pbbs_graph* g);
 int64_t graph_m(const pbbs_graph* g);
 int graph_device(const pbbs_graph* g);
@@ -270,7 +273,7 @@ extern "C" int pbbs_check_bfs(pbbs_ctx* c, const pbbs_graph* g, int32_t source,
   r.valid = h[kBad] == 0;
   r.bad = h[kBad];
   r.first_bad = h[kBad] ? h[kFirstBad] : -1;
-  r.max_rule_bad = h[kMaxBad];
+  r.max_rule_bad = pbbs::graph_symmetric(g) ? h[kMaxBad] : -1;
   r.depth_bad = h[kDepthBad];
   r.reached = h[kReached];
   r.max_level = (int32_t)h[kMaxLevel];

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -107,10 +108,11 @@ struct pbbs_ctx {
   std::set<pbbs_graph*> graphs;
   // workspace, sized for the largest graph seen so far
   int64_t cap_n = 0;
+  int64_t cap_q = 0;                     // entries per sub-queue region w_q/w_qe hold
   int32_t* w_parent = nullptr;
   int32_t* w_depth = nullptr;
   unsigned long long* w_bits = nullptr;  // visited, fb0, fb1, fb2
-  int32_t* w_q = nullptr;                // q0 | q1, each kSubQueues regions of cap_n
+  int32_t* w_q = nullptr;                // q0 | q1, each kSubQueues regions of qcap entries
   uint32_t* w_qe = nullptr;              // qe0 | qe1, same layout
   uint32_t* w_wg_level = nullptr;        // kMaxGrid per-workgroup level counters
   BfsState* w_state = nullptr;
@@ -142,6 +144,8 @@ struct pbbs_ctx {
   int32_t* b_parent = nullptr;
   int32_t* b_depth = nullptr;
   uint8_t* b_dlev = nullptr;             // [cap_bn][64] per-vertex level table
+  int32_t* b_q = nullptr;                // union queues: q0 | q1, kSubQueues regions of cap_bn
+  uint32_t* b_qe = nullptr;
   bool b_last_args_off64 = false;
   alignas(16) unsigned char b_last_args[sizeof(MbfsArgs<int64_t>)] = {};  // host copy for stats
   std::map<std::tuple<int, int, int>, hipGraphExec_t> batch_graphs;
@@ -188,21 +192,36 @@ static void free_workspace(pbbs_ctx* c) {
   c->w_q = nullptr;
   c->w_qe = nullptr;
   c->cap_n = 0;
+  c->cap_q = 0;
 }
 
+namespace pbbs {
+// Entries per sub-queue region of the single-source frontier queues: n / div
+// over all kSubQueues regions, at least 64 each (n at most).  A frontier
+// that does not fit is expanded from its bitmap by the level that reads it.
+int64_t queue_cap(int64_t n, int32_t div) {
+  const int64_t per = (n + (int64_t)div * kSubQueues - 1) / ((int64_t)div * kSubQueues);
+  return std::min<int64_t>(n, std::max<int64_t>(kWave, per));
+}
+}  // namespace pbbs
+
 static int ensure_workspace(pbbs_ctx* c, int64_t n) {
-  if (n <= c->cap_n) return PBBS_OK;
+  const int64_t qcap = queue_cap(n, c->tuning.queue_div);
+  if (n <= c->cap_n && qcap <= c->cap_q) return PBBS_OK;
+  const int64_t cn = std::max(n, c->cap_n), cq = std::max(qcap, c->cap_q);
   free_workspace(c);
-  const int64_t nwords = (n + kWave - 1) / kWave;
-  if (hipMalloc(&c->w_parent, n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->w_depth, n * sizeof(int32_t)) != hipSuccess ||
+  const int64_t nwords = (cn + kWave - 1) / kWave;
+  if (hipMalloc(&c->w_parent, cn * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->w_depth, cn * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->w_bits, 4 * nwords * sizeof(unsigned long long)) != hipSuccess ||
-      hipMalloc(&c->w_q, 2 * kSubQueues * n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->w_qe, 2 * kSubQueues * n * sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&c->w_q, 2 * kSubQueues * cq * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->w_qe, 2 * kSubQueues * cq * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
     free_workspace(c);
     return fail(PBBS_ERR_NOMEM, "workspace allocation for n=%lld failed", (long long)n);
   }
-  c->cap_n = n;
+  c->cap_n = cn;
+  c->cap_q = cq;
   return PBBS_OK;
 }
 
@@ -216,6 +235,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->use_graphs = 1;
   t->batch_pull_div = 10;
   t->bu_row_tops = 1;
+  t->queue_div = 4;
   return PBBS_OK;
 }
 
@@ -269,6 +289,8 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   free_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
+  (void)hipFree(c->b_q);
+  (void)hipFree(c->b_qe);
   (void)hipFree(c->b_state);
   if (c->b_host_state) (void)hipHostFree(c->b_host_state);
   (void)hipFree(c->b_args);
@@ -302,11 +324,11 @@ int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
   if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
       t->bu_serial_depth < 0 || t->bu_serial_depth > 4096 || t->chunk_levels < 1 ||
       t->chunk_levels > 4096 || t->batch_pull_div < 0 || t->bu_row_tops < 0 ||
-      t->bu_row_tops > 1)
+      t->bu_row_tops > 1 || t->queue_div < 1)
     return fail(PBBS_ERR_INVALID,
                 "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
                 "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096, batch_pull_div>=0, "
-                "bu_row_tops 0 or 1)");
+                "bu_row_tops 0 or 1, queue_div>=1)");
   c->tuning = *t;
   return PBBS_OK;
 }
@@ -543,6 +565,7 @@ static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32
     return fail(PBBS_ERR_HIP, "wavefront shares: %s", hipGetErrorString(e));
   }
   g->bounds[waves] = d;
+  g->device_bytes += (waves + 1) * (int64_t)sizeof(uint32_t);
   *out = d;
   return PBBS_OK;
 }
@@ -597,6 +620,7 @@ static int ensure_pull_rows(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const int
     return fail(PBBS_ERR_HIP, "edge-slice rows: %s", hipGetErrorString(e));
   }
   g->pull_rows[waves] = d;
+  g->device_bytes += (waves + 1) * (int64_t)sizeof(int32_t);
   *out = d;
   return PBBS_OK;
 }
@@ -653,11 +677,12 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.fb[0] = c->w_bits + nwords;
   a.fb[1] = c->w_bits + 2 * nwords;
   a.fb[2] = c->w_bits + 3 * nwords;
+  const int64_t qcap = queue_cap(g->n, c->tuning.queue_div);   // <= c->cap_q
   a.q[0] = c->w_q;
-  a.q[1] = c->w_q + kSubQueues * c->cap_n;
+  a.q[1] = c->w_q + kSubQueues * qcap;
   a.qe[0] = c->w_qe;
-  a.qe[1] = c->w_qe + kSubQueues * c->cap_n;
-  a.qcap = c->cap_n;
+  a.qe[1] = c->w_qe + kSubQueues * qcap;
+  a.qcap = qcap;
   a.st = c->w_state;
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
@@ -971,24 +996,99 @@ extern "C" int pbbs_bfs_profile(pbbs_ctx* c, pbbs_graph* g, int32_t source,
                   : run_profile<uint32_t>(c, g, source, out, max_levels, count, init_ms);
 }
 
+// ------------------------------------------------------------- preparation
+static int batch_grid(pbbs_ctx* c, bool off64, bool sorted);
+
+template <typename OffT>
+static int prepare_graph(pbbs_ctx* c, pbbs_graph* g, uint32_t what) {
+  if (what & PBBS_PREPARE_SINGLE) {
+    const uint32_t* tops = nullptr;
+    int rc = ensure_tops<OffT>(c, g, &tops);
+    if (rc != PBBS_OK) return rc;
+    const int grid = level_grid(c, g->off64, level_variant(g, tops));
+    const uint32_t* bounds = nullptr;
+    rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+    if (rc != PBBS_OK) return rc;
+  }
+  if ((what & PBBS_PREPARE_BATCH) && g->symmetric) {
+    int rc = ensure_ranks<OffT>(c, g);
+    if (rc != PBBS_OK) return rc;
+    const int32_t* rows = nullptr;
+    rc = ensure_pull_rows<OffT>(c, g, (int64_t)batch_grid(c, g->off64, g->sorted) * kWavesPerBlock,
+                                &rows);
+    if (rc != PBBS_OK) return rc;
+  }
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_graph_prepare(pbbs_ctx* c, pbbs_graph* g, uint32_t what, double* ms) {
+  clear_error();
+  if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (what & ~(uint32_t)(PBBS_PREPARE_SINGLE | PBBS_PREPARE_BATCH))
+    return fail(PBBS_ERR_INVALID, "unknown prepare flags 0x%x", what);
+  HIP_TRY(hipSetDevice(c->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int rc = g->off64 ? prepare_graph<int64_t>(c, g, what) : prepare_graph<uint32_t>(c, g, what);
+  if (rc != PBBS_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (ms)
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  return PBBS_OK;
+}
+
 // --------------------------------------------------------------- batched BFS
 static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, bool need_out) {
   if (!c->b_state) {
-    HIP_TRY(hipMalloc(&c->b_state, sizeof(MbfsState)));
-    HIP_TRY(hipHostMalloc((void**)&c->b_host_state, sizeof(MbfsState), hipHostMallocDefault));
-    HIP_TRY(hipMalloc(&c->b_args, sizeof(MbfsArgs<int64_t>)));
-    HIP_TRY(hipMalloc(&c->b_sources, kMaxSources * sizeof(int32_t)));
+    // all four control buffers or none: b_state marks the set as present
+    MbfsState* st = nullptr;
+    MbfsState* hst = nullptr;
+    void* args = nullptr;
+    int32_t* src = nullptr;
+    if (hipMalloc(&st, sizeof(MbfsState)) != hipSuccess ||
+        hipHostMalloc((void**)&hst, sizeof(MbfsState), hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(&args, sizeof(MbfsArgs<int64_t>)) != hipSuccess ||
+        hipMalloc(&src, kMaxSources * sizeof(int32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(st);
+      if (hst) (void)hipHostFree(hst);
+      (void)hipFree(args);
+      (void)hipFree(src);
+      return fail(PBBS_ERR_NOMEM, "batched control buffers failed");
+    }
+    c->b_host_state = hst;
+    c->b_args = args;
+    c->b_sources = src;
+    c->b_state = st;
   }
   if (n > c->cap_bn) {
     (void)hipFree(c->b_masks);
-    c->b_masks = nullptr;
-    c->cap_bn = 0;
     (void)hipFree(c->b_dlev);
+    (void)hipFree(c->b_q);
+    (void)hipFree(c->b_qe);
+    c->b_masks = nullptr;
     c->b_dlev = nullptr;
+    c->b_q = nullptr;
+    c->b_qe = nullptr;
+    c->cap_bn = 0;
+    // the union frontier's sub-queues hold n entries each: batched push
+    // levels have no bitmap fallback, so a sub-queue must never fill
     if (hipMalloc(&c->b_masks, (5 * n + 4 * ((n + kWave - 1) / kWave)) *
                                    sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc(&c->b_dlev, n * kMaxSources) != hipSuccess)
+        hipMalloc(&c->b_dlev, n * kMaxSources) != hipSuccess ||
+        hipMalloc(&c->b_q, 2 * kSubQueues * n * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->b_qe, 2 * kSubQueues * n * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(c->b_masks);
+      (void)hipFree(c->b_dlev);
+      (void)hipFree(c->b_q);
+      (void)hipFree(c->b_qe);
+      c->b_masks = nullptr;
+      c->b_dlev = nullptr;
+      c->b_q = nullptr;
+      c->b_qe = nullptr;
       return fail(PBBS_ERR_NOMEM, "batched mask buffers for n=%lld failed", (long long)n);
+    }
     c->cap_bn = n;
   }
   if (need_out && out_entries > c->cap_bout) {
@@ -1058,11 +1158,11 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.dlev = c->b_dlev;
   for (int j = 0; j < 4; ++j) a.fr[j] = c->b_masks + (j + 1) * n;
   for (int j = 0; j < 4; ++j) a.fbm[j] = c->b_masks + 5 * n + j * ((n + kWave - 1) / kWave);
-  a.q[0] = c->w_q;
-  a.q[1] = c->w_q + kSubQueues * c->cap_n;
-  a.qe[0] = c->w_qe;
-  a.qe[1] = c->w_qe + kSubQueues * c->cap_n;
-  a.qcap = c->cap_n;
+  a.q[0] = c->b_q;
+  a.q[1] = c->b_q + kSubQueues * c->cap_bn;
+  a.qe[0] = c->b_qe;
+  a.qe[1] = c->b_qe + kSubQueues * c->cap_bn;
+  a.qcap = c->cap_bn;
   a.st = c->b_state;
   a.wg_level = c->w_wg_level;
   a.host_status = c->host_status;
@@ -1160,8 +1260,7 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
   HIP_TRY(hipSetDevice(c->device));
   const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
   const int64_t entries = (int64_t)nsrc * g->n;
-  int rc = ensure_workspace(c, g->n);
-  if (rc == PBBS_OK) rc = ensure_batch_workspace(c, g->n, entries, !(dev_out && parents && depths));
+  int rc = ensure_batch_workspace(c, g->n, entries, !(dev_out && parents && depths));
   if (rc != PBBS_OK) return rc;
   int32_t* dp = (dev_out && parents) ? parents : c->b_parent;
   int32_t* dd = (dev_out && depths) ? depths : c->b_depth;
@@ -1192,12 +1291,74 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
 // ------------------------------------------------- entry points for the server
 namespace pbbs {
 
-int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc) {
+// Device bytes a context allocates for one run on an n-vertex graph: the
+// single-source workspace (nsrc <= 1: parent, depth, four bitmaps, two
+// frontier queues of queue_cap entries per sub-queue, with their edge
+// prefixes) or the batched one (nsrc >= 2: masks, bitmaps, level table,
+// union queues of n entries per sub-queue, [nsrc][n] parents and depths).
+int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc, int32_t queue_div) {
   const int64_t nwords = (n + kWave - 1) / kWave;
-  int64_t b = 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * n * 8;
-  if (nsrc > 0) b += 5 * n * 8 + 4 * nwords * 8 + n * kMaxSources + 2 * (int64_t)nsrc * n * 4;
+  if (nsrc <= 1)
+    return 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * queue_cap(n, queue_div) * 8;
+  return 5 * n * 8 + 4 * nwords * 8 + n * kMaxSources + 2 * (int64_t)kSubQueues * n * 8 +
+         2 * (int64_t)nsrc * n * 4;
+}
+
+// Device bytes the context's workspace buffers hold now (they grow with the
+// largest run and are kept until ctx_trim or the context's end).
+int64_t ctx_workspace_bytes(const pbbs_ctx* c) {
+  const int64_t nwords = (c->cap_n + kWave - 1) / kWave;
+  const int64_t bwords = (c->cap_bn + kWave - 1) / kWave;
+  int64_t b = 0;
+  if (c->w_parent) b += 8 * c->cap_n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * c->cap_q * 8;
+  if (c->b_masks)
+    b += 5 * c->cap_bn * 8 + 4 * bwords * 8 + c->cap_bn * kMaxSources +
+         2 * (int64_t)kSubQueues * c->cap_bn * 8;
+  if (c->b_parent) b += 2 * c->cap_bout * 4;
   return b;
 }
+
+// Device bytes the context's workspace will hold after a run of nsrc
+// sources on an n-vertex graph (buffers only grow).
+int64_t ctx_workspace_bytes_after(const pbbs_ctx* c, int64_t n, int32_t nsrc) {
+  int64_t cn = c->w_parent ? c->cap_n : 0, cq = c->w_parent ? c->cap_q : 0;
+  int64_t bn = c->b_masks ? c->cap_bn : 0, bo = c->b_parent ? c->cap_bout : 0;
+  if (nsrc <= 1) {
+    cn = std::max(cn, n);
+    cq = std::max(cq, queue_cap(n, c->tuning.queue_div));
+  } else {
+    bn = std::max(bn, n);
+    bo = std::max(bo, (int64_t)nsrc * n);
+  }
+  const int64_t nwords = (cn + kWave - 1) / kWave, bwords = (bn + kWave - 1) / kWave;
+  int64_t b = 0;
+  if (cn) b += 8 * cn + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * cq * 8;
+  if (bn) b += 5 * bn * 8 + 4 * bwords * 8 + bn * kMaxSources + 2 * (int64_t)kSubQueues * bn * 8;
+  if (bo) b += 2 * bo * 4;
+  return b;
+}
+
+// Frees the context's workspace buffers (the stream must be idle).
+void ctx_trim(pbbs_ctx* c) {
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  free_workspace(c);
+  (void)hipFree(c->b_masks);
+  (void)hipFree(c->b_dlev);
+  (void)hipFree(c->b_q);
+  (void)hipFree(c->b_qe);
+  (void)hipFree(c->b_parent);
+  (void)hipFree(c->b_depth);
+  c->b_masks = nullptr;
+  c->b_dlev = nullptr;
+  c->b_q = nullptr;
+  c->b_qe = nullptr;
+  c->b_parent = c->b_depth = nullptr;
+  c->cap_bn = 0;
+  c->cap_bout = 0;
+}
+
+int32_t ctx_queue_div(const pbbs_ctx* c) { return c->tuning.queue_div; }
 
 hipStream_t ctx_stream(pbbs_ctx* c) { return c->stream; }
 int ctx_device(const pbbs_ctx* c) { return c->device; }
@@ -1220,11 +1381,13 @@ int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc
     if (sources[i] < 0 || (int64_t)sources[i] >= g->n)
       return fail(PBBS_ERR_INVALID, "source %d outside the graph", sources[i]);
   HIP_TRY(hipSetDevice(c->device));
-  int rc = ensure_workspace(c, g->n);
-  if (rc != PBBS_OK) return rc;
-  if (nsrc == 1)
+  int rc;
+  if (nsrc == 1) {
+    rc = ensure_workspace(c, g->n);
+    if (rc != PBBS_OK) return rc;
     return g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth)
                     : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth);
+  }
   if (!g->symmetric) return fail(PBBS_ERR_INVALID, "batched BFS needs a symmetric graph");
   rc = ensure_batch_workspace(c, g->n, (int64_t)nsrc * g->n, true);
   if (rc != PBBS_OK) return rc;

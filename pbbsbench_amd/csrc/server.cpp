@@ -8,6 +8,12 @@
 // the level kernels never wait on other workgroups, so any mix of concurrent
 // runs completes.  Results are copied into the caller's buffers through
 // pinned staging, and each carries its arrival-to-ready latency.
+//
+// HBM accounting: the budget covers the resident graphs with their search
+// data (built when a graph is added) plus every slot's workspace buffers as
+// they stand.  A slot keeps its buffers between runs (they only grow), so a
+// run is charged what the slot will hold after it; when that does not fit,
+// the slot's buffers, and then idle slots' buffers, are freed first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,7 +32,11 @@ using pbbs::fail;
 
 // Internal engine entry points shared with the server (engine.cpp).
 namespace pbbs {
-int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc);
+int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc, int32_t queue_div);
+int64_t ctx_workspace_bytes(const pbbs_ctx* c);
+int64_t ctx_workspace_bytes_after(const pbbs_ctx* c, int64_t n, int32_t nsrc);
+void ctx_trim(pbbs_ctx* c);
+int32_t ctx_queue_div(const pbbs_ctx* c);
 int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc);
 int async_poll(pbbs_ctx* c, bool* done);
 int async_outputs(pbbs_ctx* c, const int32_t** parent, const int32_t** depth);
@@ -61,7 +71,7 @@ struct Slot {
   int32_t graph = -1;
   pbbs_graph* g = nullptr;  // captured under the lock: add_graph may grow the table
   int64_t n = 0;
-  int64_t reserved = 0;  // HBM bytes charged to this slot's run
+  int64_t retained = 0;  // HBM bytes the slot's context holds (charged to the budget)
   int32_t* pin_parent = nullptr;
   int32_t* pin_depth = nullptr;
   int64_t pin_cap = 0;
@@ -78,7 +88,9 @@ struct pbbs_server {
   std::vector<Slot> slots;
   int32_t max_batch = 64;
   int64_t budget = 0;
-  int64_t in_use = 0;
+  int64_t graph_bytes = 0;   // resident graphs with their search data
+  int64_t max_run = 0;       // the largest single-source workspace of a resident graph
+  int64_t in_use = 0;        // graph_bytes + every slot's retained bytes
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<Request> pending;
@@ -108,11 +120,19 @@ void finish(pbbs_server* s, Slot& sl, int status) {
     s->completed.push_back(res);
   }
   s->running -= (int64_t)sl.reqs.size();
-  s->in_use -= sl.reserved;
   sl.reqs.clear();
-  sl.reserved = 0;
   sl.phase = 0;
   s->cv_done.notify_all();
+}
+
+// Frees a slot's device workspace and pinned staging; the caller has taken
+// its bytes off in_use.  The slot is idle (worker thread only).
+void trim_slot(Slot& sl) {
+  pbbs::ctx_trim(sl.ctx);
+  if (sl.pin_parent) (void)hipHostFree(sl.pin_parent);
+  if (sl.pin_depth) (void)hipHostFree(sl.pin_depth);
+  sl.pin_parent = sl.pin_depth = nullptr;
+  sl.pin_cap = 0;
 }
 
 int ensure_pinned(Slot& sl, int64_t entries) {
@@ -131,9 +151,13 @@ int ensure_pinned(Slot& sl, int64_t entries) {
 }
 
 // Takes the oldest waiting query and, for a symmetric graph, up to
-// max_batch - 1 more for the same graph, if the HBM budget allows the run.
+// max_batch - 1 more for the same graph, if the HBM budget allows the run:
+// as the slot's buffers stand, else with them freed first, else with idle
+// slots' buffers freed too; the batch shrinks until one of these fits.  A
+// lone query always fits an idle server (add_graph checks that).
 bool admit(pbbs_server* s, Slot& sl) {
   std::vector<Request> take;
+  std::vector<Slot*> trim;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->pending.empty()) return false;
@@ -141,17 +165,31 @@ bool admit(pbbs_server* s, Slot& sl) {
     pbbs_graph* g = s->graphs[gid];
     const int64_t n = pbbs::graph_n(g);
     const int cap = pbbs::graph_symmetric(g) ? s->max_batch : 1;
+    const int32_t qdiv = pbbs::ctx_queue_div(sl.ctx);
     int want = 0;
     for (const Request& r : s->pending)
       if (r.graph == gid && want < cap) ++want;
-    // Shrink the batch to what the budget holds now; a lone query always fits
-    // an empty server (submit rejects graphs that do not).
-    int64_t need = pbbs::bfs_workspace_bytes(n, want > 1 ? want : 0);
-    while (want > 1 && s->in_use + need > s->budget) {
-      --want;
-      need = pbbs::bfs_workspace_bytes(n, want > 1 ? want : 0);
+    int64_t idle = 0;   // bytes other idle slots hold
+    for (Slot& o : s->slots)
+      if (&o != &sl && o.phase == 0) idle += o.retained;
+    int how = -1;       // 0 keep the buffers, 1 free this slot's, 2 also idle slots'
+    int64_t after = 0;
+    for (; want >= 1 && how < 0; --want) {
+      const int64_t keep = pbbs::ctx_workspace_bytes_after(sl.ctx, n, want);
+      const int64_t fresh = pbbs::bfs_workspace_bytes(n, want, qdiv);
+      if (s->in_use - sl.retained + keep <= s->budget) {
+        how = 0;
+        after = keep;
+      } else if (s->in_use - sl.retained + fresh <= s->budget) {
+        how = 1;
+        after = fresh;
+      } else if (s->in_use - sl.retained - idle + fresh <= s->budget) {
+        how = 2;
+        after = fresh;
+      }
+      if (how >= 0) break;
     }
-    if (s->in_use + need > s->budget && s->in_use > 0) return false;  // wait for memory
+    if (how < 0) return false;   // wait for running slots to finish
     for (auto it = s->pending.begin(); it != s->pending.end() && (int)take.size() < want;) {
       if (it->graph == gid) {
         take.push_back(*it);
@@ -160,13 +198,22 @@ bool admit(pbbs_server* s, Slot& sl) {
         ++it;
       }
     }
-    s->in_use += need;
+    if (how >= 1) trim.push_back(&sl);
+    if (how == 2)
+      for (Slot& o : s->slots)
+        if (&o != &sl && o.phase == 0 && o.retained > 0) trim.push_back(&o);
+    for (Slot* t : trim) {
+      s->in_use -= t->retained;
+      t->retained = 0;
+    }
+    s->in_use += after - sl.retained;
+    sl.retained = after;
     s->running += (int64_t)take.size();
-    sl.reserved = need;
     sl.graph = gid;
     sl.g = g;
     sl.n = n;
   }
+  for (Slot* t : trim) trim_slot(*t);
   const auto now = Clock::now();
   for (Request& r : take) r.admitted = now;
   sl.reqs = std::move(take);
@@ -175,12 +222,38 @@ bool admit(pbbs_server* s, Slot& sl) {
   int rc = ensure_pinned(sl, (int64_t)src.size() * sl.n);
   if (rc == PBBS_OK)
     rc = pbbs::async_begin(sl.ctx, sl.g, src.data(), (int32_t)src.size());
+  {
+    // charge what the context holds now (equal to the estimate unless an
+    // allocation failed)
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int64_t held = pbbs::ctx_workspace_bytes(sl.ctx);
+    s->in_use += held - sl.retained;
+    sl.retained = held;
+  }
   if (rc != PBBS_OK) {
     finish(s, sl, rc);
     return true;
   }
   sl.phase = 1;
   return true;
+}
+
+// Frees idle slots' buffers while the server holds more than its budget
+// (a graph added while slots kept buffers from earlier runs).
+void trim_over_budget(pbbs_server* s) {
+  std::vector<Slot*> trim;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    for (Slot& o : s->slots) {
+      if (s->in_use <= s->budget) break;
+      if (o.phase == 0 && o.retained > 0) {
+        s->in_use -= o.retained;
+        o.retained = 0;
+        trim.push_back(&o);
+      }
+    }
+  }
+  for (Slot* t : trim) trim_slot(*t);
 }
 
 // Level loop finished: stats, then copy outputs to pinned staging.
@@ -250,6 +323,7 @@ void worker_loop(pbbs_server* s) {
         }
       }
     }
+    trim_over_budget(s);
     for (Slot& sl : s->slots)
       if (sl.phase == 0) admit(s, sl);
     for (Slot& sl : s->slots) active |= sl.phase != 0;
@@ -335,9 +409,32 @@ int pbbs_server_add_graph(pbbs_server* s, int64_t n, int64_t m, const void* offs
   pbbs::clear_error();
   if (!s || !graph_id) return fail(PBBS_ERR_INVALID, "NULL argument");
   pbbs_graph* g = nullptr;
-  const int rc = pbbs_graph_upload(s->main, n, m, offsets, adj, flags, &g);
+  int rc = pbbs_graph_upload(s->main, n, m, offsets, adj, flags, &g);
   if (rc != PBBS_OK) return rc;
+  // search data now, so runs allocate nothing per graph and the bytes are known
+  pbbs_graph_info info{};
+  rc = pbbs_graph_info_get(g, &info);
+  if (rc == PBBS_OK)
+    rc = pbbs_graph_prepare(s->main, g,
+                            PBBS_PREPARE_SINGLE | (info.symmetric ? PBBS_PREPARE_BATCH : 0u),
+                            nullptr);
+  if (rc == PBBS_OK) rc = pbbs_graph_info_get(g, &info);
+  if (rc != PBBS_OK) {
+    pbbs_graph_free(g);
+    return rc;
+  }
   std::lock_guard<std::mutex> lk(s->mu);
+  const int64_t run = pbbs::bfs_workspace_bytes(n, 1, pbbs::ctx_queue_div(s->main));
+  if (s->graph_bytes + info.device_bytes + std::max(s->max_run, run) > s->budget) {
+    pbbs_graph_free(g);
+    return fail(PBBS_ERR_LIMIT,
+                "graph of %lld HBM bytes plus a run's workspace does not fit the server budget "
+                "(%lld of %lld bytes held by graphs)",
+                (long long)info.device_bytes, (long long)s->graph_bytes, (long long)s->budget);
+  }
+  s->graph_bytes += info.device_bytes;
+  s->in_use += info.device_bytes;
+  s->max_run = std::max(s->max_run, run);
   s->graphs.push_back(g);
   *graph_id = (int32_t)s->graphs.size() - 1;
   return PBBS_OK;
@@ -354,8 +451,6 @@ int pbbs_server_submit(pbbs_server* s, int32_t graph_id, int32_t source, int32_t
       return fail(PBBS_ERR_INVALID, "unknown graph %d", graph_id);
     if (source < 0 || (int64_t)source >= pbbs::graph_n(s->graphs[graph_id]))
       return fail(PBBS_ERR_INVALID, "source %d outside graph %d", source, graph_id);
-    if (pbbs::bfs_workspace_bytes(pbbs::graph_n(s->graphs[graph_id]), 0) > s->budget)
-      return fail(PBBS_ERR_LIMIT, "graph %d needs more HBM than the server budget", graph_id);
     r.ticket = s->next_ticket++;
     r.graph = graph_id;
     r.source = source;

@@ -100,6 +100,11 @@ def test_permuted_rmat(ctx):
     dict(bu_row_tops=0),                        # bottom-up reads row tops from the CSR
     dict(bu_row_tops=0, td_to_bu_div=1_000_000, bu_to_td_div=1_000_000),
     dict(bu_row_tops=0, bu_serial_depth=0, grid_blocks=3),
+    # frontier queues of 64 entries per sub-queue: larger frontiers overflow and
+    # the level after them expands the frontier bitmap
+    dict(queue_div=1_000_000),
+    dict(queue_div=1_000_000, td_to_bu_div=0),
+    dict(queue_div=1_000_000, td_to_bu_div=0, grid_blocks=7, use_graphs=0),
 ])
 def test_tuning_invariance(tune):
     say("tuning", tune)

@@ -88,6 +88,22 @@ def test_check_max_rule(ctx):
     dg.free()
 
 
+def test_check_directed_graph_skips_max_rule(ctx):
+    """On a graph that is not symmetric a row lists out-neighbours, not the
+    in-neighbours the max rule needs: the checker reports -1 for it, and
+    the BFSCheck rule itself still matches the oracle's verdict."""
+    g, wp, wd = golden("directed6")
+    dg = ctx.upload(g)
+    assert dg.info["symmetric"] == 0
+    p, d, _ = dg.bfs(0)
+    assert np.array_equal(p, wp) and np.array_equal(d, wd)
+    r = dg.check(0, wp, wd)
+    assert r["max_rule_bad"] == -1 and r["depth_bad"] == 0
+    want = oracle.check_bfs(g.offsets, g.adj, 0, wp)
+    assert (r["valid"] == 1) == (want == -1) and r["first_bad"] == want, (r, want)
+    dg.free()
+
+
 @pytest.mark.parametrize("wide", [False, True])
 def test_check_large_and_batched(ctx, wide):
     """RMAT 2^14: a real BFS passes, a shuffled parent fails the max rule, and

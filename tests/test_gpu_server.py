@@ -66,7 +66,8 @@ def test_server_mix_equals_alone():
             say(f"submit {i}: graph {gid} source {s}")
             queries[srv.submit(gid, s, depth=(i % 3 != 0))] = (gid, s)
         got = srv.drain(queries)
-        assert srv.load() == {"queued": 0, "running": 0, "hbm_in_use": 0}
+        ld = srv.load()
+        assert ld["queued"] == 0 and ld["running"] == 0 and ld["hbm_in_use"] > 0, ld
     batched = check_results(graphs, got, queries, cache)
     assert batched > 0, "a burst of queries on one graph should be merged"
     for t, (gid, _) in queries.items():
@@ -132,17 +133,62 @@ def test_server_concurrent_submitters():
     check_results(graphs, got, queries, cache)
 
 
-def test_server_hbm_budget_serialises():
-    """A budget that holds one run at a time still answers every query."""
-    g = pb.rmat(10, 16 * 2 ** 10, 0.57, 0.19, 0.19, seed=4)
-    n = g.n
-    one_run = 8 * n + 4 * ((n + 63) // 64) * 8 + 2 * 8 * n * 8   # single-source workspace
+def hip_free_bytes():
+    """Free device memory as the HIP runtime reports it (hipMemGetInfo, found
+    through the library's own HIP runtime dependency)."""
+    import ctypes as C
+    from pbbsbench_amd import _lib
+    fn = _lib.lib().hipMemGetInfo
+    fn.restype = C.c_int
+    free, total = C.c_size_t(), C.c_size_t()
+    assert fn(C.byref(free), C.byref(total)) == 0
+    return free.value
+
+
+def test_server_hbm_budget_bounds_device_memory(ctx):
+    """A budget that holds the graphs plus one batched run of 8 sources: the
+    server's charge never exceeds it, queries are still all answered, and
+    the device memory the server takes stays within the budget (plus each
+    context's fixed control blocks)."""
+    graphs = [pb.rmat(16, 16 * 2 ** 16, 0.57, 0.19, 0.19, seed=4),
+              pb.rmat(15, 16 * 2 ** 15, 0.57, 0.19, 0.19, seed=5)]
+    # what the server will charge for each graph: the graph and its search data
+    gbytes = 0
+    for g in graphs:
+        dg = ctx.upload(g)
+        dg.prepare()
+        gbytes += dg.info["device_bytes"]
+        dg.free()
+    n = graphs[0].n
+    batched8 = 5 * n * 8 + 4 * ((n + 63) // 64) * 8 + 64 * n + 2 * 8 * n * 8 + 2 * 8 * n * 4
+    budget = gbytes + batched8 + (1 << 20)
+    slots = 4
     cache = {}
-    with pb.Server(0, slots=4, max_batch=1, hbm_budget=one_run + 1) as srv:
-        gid = srv.add_graph(g)
-        queries = {srv.submit(gid, s): (gid, s) for s in range(0, 1000, 37)}
-        got = srv.drain(queries)
-    check_results([g], got, queries, cache)
+    free0 = hip_free_bytes()
+    peak_charge, peak_drop = 0, 0
+    with pb.Server(0, slots=slots, max_batch=8, hbm_budget=budget) as srv:
+        ids = [srv.add_graph(g) for g in graphs]
+        rng = np.random.default_rng(3)
+        queries = {}
+        for i in range(160):
+            gi = int(rng.integers(0, 2))
+            s = int(rng.integers(0, graphs[gi].n))
+            queries[srv.submit(ids[gi], s, depth=(i % 2 == 0))] = (gi, s)
+            if i % 16 == 0:
+                peak_charge = max(peak_charge, srv.load()["hbm_in_use"])
+                peak_drop = max(peak_drop, free0 - hip_free_bytes())
+        got = {}
+        while len(got) < len(queries):
+            for r, p, d in srv.poll(wait_ms=20):
+                got[r["ticket"]] = (r, p, d)
+            peak_charge = max(peak_charge, srv.load()["hbm_in_use"])
+            peak_drop = max(peak_drop, free0 - hip_free_bytes())
+    say(f"budget {budget}, peak charge {peak_charge}, peak device drop {peak_drop}")
+    assert peak_charge <= budget
+    # contexts' fixed blocks (control state, per-workgroup counters, argument
+    # blocks, captured launch graphs): at most 16 MiB each, server main + slots
+    assert peak_drop <= budget + (slots + 1) * (16 << 20), (peak_drop, budget)
+    check_results(graphs, got, {t: (ids[gi], s) for t, (gi, s) in queries.items()}, cache)
 
 
 def test_server_rejects_bad_queries():
@@ -153,9 +199,8 @@ def test_server_rejects_bad_queries():
             srv.submit(gid, g.n)
         with pytest.raises(pb.PbbsError):
             srv.submit(gid + 1, 0)
-    with pb.Server(0, hbm_budget=1024) as srv:
-        gid = srv.add_graph(g)
+    with pb.Server(0, hbm_budget=1024) as srv:   # the graph itself does not fit
         with pytest.raises(pb.PbbsError):
-            srv.submit(gid, 0)
+            srv.add_graph(g)
     with pytest.raises(pb.PbbsError):
         pb.Server(0, max_batch=65)
