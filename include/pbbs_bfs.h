@@ -89,6 +89,14 @@ typedef struct {
                                (default 4); a level whose next frontier does not fit stores
                                the rest in the frontier bitmap only, and the level after it
                                expands that bitmap (same outputs, bounded workspace) */
+    int32_t persist;        /* 1 = a single-source BFS runs every level in one persistent
+                               launch whose workgroups meet at a grid barrier between levels
+                               (default); one such launch per device at a time, others run one
+                               level per launch.  0 = always one level per launch */
+    int32_t persist_wait_us;/* longest wait at a barrier of a persistent launch (default
+                               200000); a launch that waits longer (its grid could not gather,
+                               e.g. on a device shared with other processes) gives up and the
+                               BFS runs again one level per launch, with the same outputs */
 } pbbs_tuning;
 
 int pbbs_tuning_get(const pbbs_ctx* ctx, pbbs_tuning* out);
@@ -176,6 +184,26 @@ int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
  * (bench/BFS.h:11) over many starts. */
 int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t nsrc,
                    int32_t* parents, int32_t* depths, uint32_t out_flags, pbbs_bfs_stats* stats);
+
+/* ---- several graphs in one launch --------------------------------------- */
+#define PBBS_MAX_MULTI 64
+
+/* Single-source BFS on each of njobs graphs (1 <= njobs <= 64; graphs may
+ * repeat and may differ in size) in one persistent launch: each job gets a
+ * share of the device's resident workgroups in proportion to n + m and runs
+ * its levels on them, meeting its own barrier, so jobs of different depths
+ * proceed independently.  Jobs are grouped by offset width and row order,
+ * one launch per group.  parents[j] / depths[j] (either array, or any entry,
+ * may be NULL) receive job j's outputs, n_j entries each; with
+ * PBBS_OUT_DEVICE they are device pointers.  stats[j] is job j's (device_ms
+ * is the whole launch's).  Every output is bit-identical to pbbs_bfs on
+ * graphs[j] from sources[j].  A job whose workgroups could not gather, or
+ * all jobs when another persistent launch holds the device, runs alone
+ * instead, with the same outputs.  No reference counterpart: it runs the
+ * reference's BFS(start, G) (bench/BFS.h:11) on several graphs at once. */
+int pbbs_bfs_multi(pbbs_ctx* ctx, int32_t njobs, pbbs_graph* const* graphs,
+                   const int32_t* sources, int32_t* const* parents, int32_t* const* depths,
+                   uint32_t out_flags, pbbs_bfs_stats* stats);
 
 /* Device pointers of the context's own output buffers: those pbbs_bfs
  * (batched = 0; n entries each) or pbbs_bfs_batch (batched = 1; [k][n]) last

@@ -37,6 +37,8 @@ class Tuning(C.Structure):
         ("batch_pull_div", C.c_int32),
         ("bu_row_tops", C.c_int32),
         ("queue_div", C.c_int32),
+        ("persist", C.c_int32),
+        ("persist_wait_us", C.c_int32),
     ]
 
 
@@ -149,6 +151,8 @@ SIGNATURES = [
     ("pbbs_graph_info_get", C.c_int, [_P, C.POINTER(GraphInfo)]),
     ("pbbs_graph_free", C.c_int, [_P]),
     ("pbbs_graph_prepare", C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_double)]),
+    ("pbbs_bfs_multi", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.c_uint32,
+                                 C.POINTER(BfsStats)]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_batch", C.c_int,
      [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),

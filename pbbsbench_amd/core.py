@@ -244,6 +244,23 @@ class Context:
         g.n, g.m = i["n"], i["m"]
         return g
 
+    def bfs_multi(self, graphs, sources, depth=True):
+        """Single-source BFS on each (DeviceGraph, source) pair in one
+        persistent launch (pbbs_bfs_multi).  Returns a list of (parent, depth
+        or None, stats) in job order; each equals graphs[j].bfs(sources[j])."""
+        J = len(graphs)
+        if J != len(sources):
+            raise ValueError("one source per graph")
+        par = [np.empty(g.n, dtype=np.int32) for g in graphs]
+        dep = [np.empty(g.n, dtype=np.int32) for g in graphs] if depth else None
+        gh = (C.c_void_p * J)(*[g._h for g in graphs])
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        pp = (C.c_void_p * J)(*[a.ctypes.data for a in par])
+        dp = (C.c_void_p * J)(*[a.ctypes.data for a in dep]) if depth else None
+        st = (_lib.BfsStats * J)()
+        check(lib().pbbs_bfs_multi(self._h, J, gh, _ptr(src), pp, dp, 0, st), "BFS multi")
+        return [(par[j], dep[j] if depth else None, st[j].as_dict()) for j in range(J)]
+
     def gen_rmat(self, scale, draws, a, b, c, seed=1, permute_seed=0, wide_offsets=False):
         """rmat(...) built on the GPU, straight into HBM; the same graph bit for
         bit as ctx.upload(rmat(...)) (testData/graphData/rMatGraph.C:33-79)."""

@@ -13,6 +13,7 @@ constexpr int kSubQueues = 8;         // frontier sub-queues (one append counter
 constexpr int kWaveBuf = 512;         // LDS staging entries per wavefront
 constexpr int kMaxGrid = 1 << 16;     // level-kernel workgroups at most
 constexpr int kLevelLogMax = 4096;    // per-level records kept for stats/tuning
+constexpr int kBarGroups = 32;        // first-tier counters of the persistent launch's grid barrier
 
 enum : int32_t { kTopDown = 0, kBottomUp = 1 };
 
@@ -21,6 +22,12 @@ struct LevelRecord {
   int32_t mode;
   uint32_t count;   // |F_k| (vertices of degree > 0)
   uint64_t edges;   // sum of degrees over F_k
+};
+
+// A counter alone on its 128-byte line (same-line atomics serialise).
+struct alignas(128) LineCounter {
+  unsigned v;
+  unsigned pad[31];
 };
 
 // Device-resident control block of one BFS run.
@@ -46,6 +53,15 @@ struct BfsState {
   int32_t pad;
   unsigned long long reached;        // n_r
   unsigned long long edges_reached;  // m_r
+  // grid barrier of a persistent launch: workgroup b arrives on
+  // bar_grp[b % kBarGroups]; a group's last arrival arrives on bar_top, whose
+  // last arrival bumps bar_gen.  abort is set by a workgroup whose wait ran
+  // out; every workgroup then leaves the launch.
+  LineCounter bar_grp[kBarGroups];
+  LineCounter bar_top;
+  LineCounter bar_gen;
+  int32_t abort;
+  int32_t pad2;
   LevelRecord log[kLevelLogMax];
 };
 
@@ -69,7 +85,8 @@ struct BfsArgs {
   uint32_t* qe[2];             // per entry: exclusive degree prefix within its sub-queue
   BfsState* st;
   uint32_t* wg_level;          // per-workgroup level counter (kMaxGrid entries)
-  int32_t* host_status;        // pinned host memory: [0] done, [1] levels started
+  int32_t* host_status;        // pinned host memory: [0] done, [1] levels started,
+                               // [2] a persistent launch gave up waiting at its barrier
   BfsState* host_state;        // pinned host memory: the final counters, written by the
                                // kernel that ends the BFS before it sets host_status[0]
   const uint32_t* bu_bounds;   // first-word index of each wavefront's share in the first
@@ -86,7 +103,14 @@ struct BfsArgs {
   int32_t symmetric;
   int32_t num_cus;             // CUs of the device (workgroup slot = blockIdx / num_cus)
   int32_t small_rows;          // 1 if no row has more than 64 entries (top-down slices by entry)
+  long long wait_ticks;        // persistent launch: longest barrier wait (device wall-clock
+                               // ticks) before the launch gives up
 };
+
+// Jobs of a multi-graph launch (several single-source BFS runs, each on its
+// own graph, in one launch): an array of their BfsArgs and the first
+// workgroup of each (njobs + 1 entries, the last one the grid size).
+constexpr int kMaxMultiJobs = 64;
 
 // ----------------------------------------------------------- batched BFS
 // Up to 64 sources in one traversal: bit i of a per-vertex 64-bit mask

@@ -246,6 +246,8 @@ struct LevelView {
   long long n;
   long long nwords;
   long long qcap;
+  long long gw;                         // this wavefront's index in the level's grid
+  long long W;                          // wavefronts in the level's grid
   int k;
   int serial_depth;
 };
@@ -464,8 +466,8 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
 template <typename OffT>
 __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                          unsigned long long E) {
-  const unsigned long long W = (unsigned long long)num_waves();
-  const unsigned long long gw = (unsigned long long)global_wave();
+  const unsigned long long W = (unsigned long long)a.W;
+  const unsigned long long gw = (unsigned long long)a.gw;
   if (a.small_rows) {
     // Every row has at most 64 entries: slices of equal entry counts are
     // within one window of balanced, and a slice's first entry is known, so
@@ -852,8 +854,8 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   const gptr<const unsigned long long> fbc64 = (gptr<const unsigned long long>)a.fb_cur;
-  const long long W = num_waves();
-  const long long gw = global_wave();
+  const long long W = a.W;
+  const long long gw = a.gw;
   const long long b0 = a.bu_bounds ? (long long)a.bu_bounds[gw] : a.nwords * gw / W;
   const long long b1 = a.bu_bounds ? (long long)a.bu_bounds[gw + 1] : a.nwords * (gw + 1) / W;
   unsigned nq = 0, npark = 0;
@@ -922,8 +924,8 @@ __device__ void td_from_bitmap(const LevelView<OffT>& a, StageBuf& sb, Stage& st
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   const gptr<const unsigned long long> fb = (gptr<const unsigned long long>)a.fb_cur;
-  const long long W = num_waves();
-  const long long gw = global_wave();
+  const long long W = a.W;
+  const long long gw = a.gw;
   const long long w0 = a.nwords * gw / W, w1 = a.nwords * (gw + 1) / W;
   unsigned nq = 0;
   for (long long g0 = w0; g0 < w1; g0 += kWave) {
@@ -961,15 +963,15 @@ __device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsi
   return ((long long)cnt > A.n / A.bu_to_td_div) ? kBottomUp : kTopDown;
 }
 
-#ifndef PBBS_LEVEL_MIN_BLOCKS
-#define PBBS_LEVEL_MIN_BLOCKS 1
-#endif
+// Level k of the BFS, as workgroup vb of a grid of vg workgroups (the
+// launch's own grid, or this BFS's part of a launch shared with others).
+// Returns false if F_k is empty (the BFS is over), true once this
+// workgroup's share of level k is done.
 template <typename OffT, bool kSorted, bool kTops>
-__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
-  __shared__ StageBuf sb;
-  __shared__ BuLds<OffT> bl;
+__device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, const int k,
+                                           StageBuf& sb, BuLds<OffT>& bl, const unsigned vb,
+                                           const unsigned vg) {
   BfsState* st = A->st;
-  const int k = (int)A->wg_level[blockIdx.x];
   const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
   unsigned long long cnt = 0, E = 0;
 #pragma unroll
@@ -981,7 +983,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   if (cnt == 0) {
     // F_k has no vertex to expand: the BFS is over.  The first kernel to see
     // it closes the books; later (overshoot) launches return at once.
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !st->finished) {
+    if (vb == 0 && threadIdx.x == 0 && !st->finished) {
       const uint32_t z = st->d0[c0];
       st->reached += z;
       if (z) st->max_depth = k;
@@ -996,13 +998,13 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
       __hip_atomic_store(&A->host_status[1], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
       __hip_atomic_store(&A->host_status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
-    return;
+    return false;
   }
   PBBS_STAMP(k, 0);
   const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
   const int mode = level_mode(*A, prev, cnt, E);
   const bool qover = st->qover[c0] != 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (vb == 0 && threadIdx.x == 0) {
     st->modes[k & 3] = mode;
     st->reached += cnt + st->d0[c0];
     st->edges_reached += E;
@@ -1028,8 +1030,8 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
     const gptr<unsigned long long> vis = glob(A->visited);
     const bool fold = mode == kTopDown && A->symmetric && A->td_to_bu_div > 0;
     const long long nwords = A->nwords;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += stride) {
+    const long long stride = (long long)vg * blockDim.x;
+    for (long long w = (long long)vb * blockDim.x + threadIdx.x; w < nwords; w += stride) {
       z[w] = 0;
       if (fold) {
         const unsigned long long fw = f[w];
@@ -1037,7 +1039,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
       }
     }
   }
-  const int sq = blockIdx.x % kSubQueues;
+  const int sq = (int)(vb % kSubQueues);
   LevelView<OffT> a;
   a.off = glob(A->off);
   a.adj = glob(A->adj);
@@ -1064,6 +1066,8 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   a.small_rows = A->small_rows;
   a.n = A->n;
   a.nwords = A->nwords;
+  a.gw = (long long)vb * kWavesPerBlock + threadIdx.x / kWave;
+  a.W = (long long)vg * kWavesPerBlock;
   a.k = k;
   a.serial_depth = A->bu_serial_depth;
   Stage stg;
@@ -1076,10 +1080,140 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
   {
     slot_priority(A->num_cus);   // bottom-up only: top-down levels measured the same with it
     bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
+    __builtin_amdgcn_s_setprio(0);
   }
   PBBS_STAMP(k, 2);
   stage_flush_block(a, sb, stg, mode == kTopDown);
-  if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
+  return true;
+}
+
+#ifndef PBBS_LEVEL_MIN_BLOCKS
+#define PBBS_LEVEL_MIN_BLOCKS 1
+#endif
+// One level per launch: the level number comes from the workgroup's own
+// counter, so a captured sequence of launches needs no host input.
+template <typename OffT, bool kSorted, bool kTops>
+__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
+  __shared__ StageBuf sb;
+  __shared__ BuLds<OffT> bl;
+  const int k = (int)A->wg_level[blockIdx.x];
+  if (level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x) && threadIdx.x == 0)
+    A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
+}
+
+// Grid barrier between the levels of a persistent launch.  Every wavefront
+// first waits for its own stores; then one thread per workgroup writes the
+// L2 back (release), arrives, waits for the generation to move, and
+// invalidates stale lines (acquire), the cache work a kernel boundary does.
+// The wait is bounded by A->wait_ticks of the device wall clock: a workgroup
+// that runs out sets st->abort (and host_status[2] for the host), and every
+// workgroup leaves at its next barrier or level start, so the launch always
+// ends; the host then runs the BFS again one level per launch.  Returns
+// false when the launch is to end.  vb / vg: this workgroup's index in the
+// grid that meets at the barrier, and that grid's size.
+__device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_ticks,
+                             const unsigned vb, const unsigned vg) {
+  __shared__ int ok;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned G = vg < (unsigned)kBarGroups ? vg : (unsigned)kBarGroups;
+    const unsigned grp = vb % G;
+    const unsigned gsize = (vg - grp + G - 1) / G;
+    const unsigned gen =
+        __hip_atomic_load(&st->bar_gen.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (atomicAdd(&st->bar_grp[grp].v, 1u) == gsize - 1) {
+      // last of its group: reset the group counter before the group can
+      // arrive again (that needs the generation to move, which needs this)
+      __hip_atomic_store(&st->bar_grp[grp].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      if (atomicAdd(&st->bar_top.v, 1u) == G - 1) {
+        __hip_atomic_store(&st->bar_top.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __hip_atomic_fetch_add(&st->bar_gen.v, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    int good = 1;
+    const long long t0 = (long long)wall_clock64();
+    while (__hip_atomic_load(&st->bar_gen.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        good = 0;
+        break;
+      }
+      if ((long long)wall_clock64() - t0 > wait_ticks) {
+        __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&host_status[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        good = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    ok = good;
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// Every level of the BFS in one launch, separated by grid barriers.  The
+// grid must fit on the device at once (the host launches at most the
+// kernel's occupancy times the CU count); a launch that cannot gather at a
+// barrier within wait_ticks gives up as described above.  On leaving, each
+// workgroup records the level it reached, as the one-level kernel does.
+template <typename OffT, bool kSorted, bool kTops>
+__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_persist_kernel(const BfsArgs<OffT>* __restrict__ A) {
+  __shared__ StageBuf sb;
+  __shared__ BuLds<OffT> bl;
+  __shared__ int stop;
+  BfsState* st = A->st;
+  int k = (int)A->wg_level[blockIdx.x];
+  for (;;) {
+    if (threadIdx.x == 0)
+      stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (stop) break;
+    if (!level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x)) break;
+    ++k;
+    if (!grid_barrier(st, A->host_status, A->wait_ticks, blockIdx.x, gridDim.x)) break;
+  }
+  if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)k;
+}
+
+// Several single-source BFS runs in one launch, each on its own graph: job
+// j's workgroups [wg_begin, wg_begin + wg_count) run its levels as the
+// persistent kernel does, meeting at a barrier of their own, so jobs of
+// different depths and sizes proceed independently.  All jobs of a launch
+// have sorted rows (kSorted: read from the CSR, no row-top records) or all
+// have unsorted rows.  The whole grid fits on
+// the device at once (the host sizes it by occupancy); a job whose barrier
+// wait runs out ends as in the persistent kernel and the host runs it again.
+template <typename OffT, bool kSorted>
+__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_multi_kernel(
+    const BfsArgs<OffT>* __restrict__ args, const uint32_t* __restrict__ wg_begin, int njobs) {
+  __shared__ StageBuf sb;
+  __shared__ BuLds<OffT> bl;
+  __shared__ int stop;
+  int lo = 0, hi = njobs - 1;   // the last job with wg_begin <= blockIdx.x
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (wg_begin[mid] <= blockIdx.x) lo = mid; else hi = mid - 1;
+  }
+  const unsigned vb = blockIdx.x - wg_begin[lo], vg = wg_begin[lo + 1] - wg_begin[lo];
+  if (vb >= vg) return;   // not part of any job (the host sizes the grid to the jobs)
+  const BfsArgs<OffT>* __restrict__ A = args + lo;
+  BfsState* st = A->st;
+  int k = (int)A->wg_level[vb];
+  for (;;) {
+    if (threadIdx.x == 0)
+      stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    if (stop) break;
+    if (!level_step<OffT, kSorted, false>(A, k, sb, bl, vb, vg)) break;
+    ++k;
+    if (!grid_barrier(st, A->host_status, A->wait_ticks, vb, vg)) break;
+  }
+  if (threadIdx.x == 0) A->wg_level[vb] = (uint32_t)k;
 }
 
 template <typename OffT>
@@ -1126,7 +1260,12 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
     st->max_depth = 0;
     st->reached = 0;
     st->edges_reached = 0;
+    for (int j = 0; j < kBarGroups; ++j) st->bar_grp[j].v = 0;
+    st->bar_top.v = 0;
+    st->bar_gen.v = 0;
+    st->abort = 0;
     __threadfence_system();
+    __hip_atomic_store(&a.host_status[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
@@ -2264,6 +2403,48 @@ const void* level_kernel_symbol(int variant) {
 }
 
 template <typename OffT>
+const void* persist_kernel_symbol(int variant) {
+  switch (variant) {
+    case kLevelSortedTops: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, true, true>);
+    case kLevelSorted: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, true, false>);
+    default: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, false, false>);
+  }
+}
+
+template <typename OffT>
+hipError_t launch_bfs_persist(const BfsArgs<OffT>* dargs, int variant, int grid,
+                              hipStream_t stream) {
+  if (variant == kLevelSortedTops)
+    hipLaunchKernelGGL((bfs_persist_kernel<OffT, true, true>), dim3(grid), dim3(kBlock), 0, stream,
+                       dargs);
+  else if (variant == kLevelSorted)
+    hipLaunchKernelGGL((bfs_persist_kernel<OffT, true, false>), dim3(grid), dim3(kBlock), 0, stream,
+                       dargs);
+  else
+    hipLaunchKernelGGL((bfs_persist_kernel<OffT, false, false>), dim3(grid), dim3(kBlock), 0,
+                       stream, dargs);
+  return hipGetLastError();
+}
+
+template <typename OffT>
+const void* multi_kernel_symbol(bool sorted) {
+  return sorted ? reinterpret_cast<const void*>(&bfs_multi_kernel<OffT, true>)
+                : reinterpret_cast<const void*>(&bfs_multi_kernel<OffT, false>);
+}
+
+template <typename OffT>
+hipError_t launch_bfs_multi(const BfsArgs<OffT>* args, const uint32_t* wg_begin, int njobs,
+                            bool sorted, int grid, hipStream_t stream) {
+  if (sorted)
+    hipLaunchKernelGGL((bfs_multi_kernel<OffT, true>), dim3(grid), dim3(kBlock), 0, stream, args,
+                       wg_begin, njobs);
+  else
+    hipLaunchKernelGGL((bfs_multi_kernel<OffT, false>), dim3(grid), dim3(kBlock), 0, stream, args,
+                       wg_begin, njobs);
+  return hipGetLastError();
+}
+
+template <typename OffT>
 hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, int variant, int grid, hipStream_t stream) {
   if (variant == kLevelSortedTops)
     hipLaunchKernelGGL((bfs_level_kernel<OffT, true, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
@@ -2470,6 +2651,11 @@ extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
                                             hipStream_t);                                        \
   template const void* level_kernel_symbol<OffT>(int);                                          \
   template hipError_t launch_bfs_level<OffT>(const BfsArgs<OffT>*, int, int, hipStream_t);       \
+  template const void* persist_kernel_symbol<OffT>(int);                                        \
+  template const void* multi_kernel_symbol<OffT>(bool);                                         \
+  template hipError_t launch_bfs_multi<OffT>(const BfsArgs<OffT>*, const uint32_t*, int, bool,   \
+                                            int, hipStream_t);                                   \
+  template hipError_t launch_bfs_persist<OffT>(const BfsArgs<OffT>*, int, int, hipStream_t);     \
   template hipError_t launch_graph_offsets<OffT>(const OffT*, long long, long long,             \
                                                  unsigned long long*, unsigned long long*, int,  \
                                                  hipStream_t);                                   \

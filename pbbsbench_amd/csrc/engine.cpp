@@ -31,6 +31,16 @@ const void* level_kernel_symbol(int variant);
 template <typename OffT>
 hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, int variant, int grid, hipStream_t stream);
 template <typename OffT>
+const void* persist_kernel_symbol(int variant);
+template <typename OffT>
+const void* multi_kernel_symbol(bool sorted);
+template <typename OffT>
+hipError_t launch_bfs_multi(const BfsArgs<OffT>* args, const uint32_t* wg_begin, int njobs,
+                            bool sorted, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_bfs_persist(const BfsArgs<OffT>* dargs, int variant, int grid,
+                              hipStream_t stream);
+template <typename OffT>
 hipError_t launch_graph_offsets(const OffT* off, long long n, long long m,
                                 unsigned long long* iso, unsigned long long* counters, int grid,
                                 hipStream_t stream);
@@ -101,6 +111,7 @@ struct pbbs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int wall_khz = 0;          // device wall-clock rate (barrier wait bound)
   pbbs_tuning tuning{};
   // graphs uploaded through this context: destroying the context detaches
   // them (they stay usable from other contexts on the device, and freeable)
@@ -131,6 +142,11 @@ struct pbbs_ctx {
     int64_t launched = 0;
     int64_t limit = 0;
     bool finalized = false;  // batched: the depth pass after the last level is queued
+    bool persist = false;    // single source: every level in one persistent launch
+    bool token = false;      // holds the device's persistent-launch token
+    int32_t source = 0;
+    int32_t* parent = nullptr;
+    int32_t* depth = nullptr;
     pbbs_graph* graph = nullptr;  // learns the launch count when the run is done
   } run;
   // batched BFS
@@ -149,6 +165,14 @@ struct pbbs_ctx {
   bool b_last_args_off64 = false;
   alignas(16) unsigned char b_last_args[sizeof(MbfsArgs<int64_t>)] = {};  // host copy for stats
   std::map<std::tuple<int, int, int>, hipGraphExec_t> batch_graphs;
+  // multi-graph launches: the jobs' workspaces, their argument blocks and
+  // first workgroups, and pinned status words and control-block copies
+  unsigned char* m_ws = nullptr;
+  int64_t m_ws_bytes = 0;
+  void* m_args = nullptr;                // BfsArgs<int64_t>[kMaxMultiJobs]
+  uint32_t* m_wg = nullptr;              // [kMaxMultiJobs + 1]
+  int32_t* m_host_status = nullptr;      // pinned: 16 words per job
+  BfsState* m_host_state = nullptr;      // pinned: one per job
 };
 
 struct pbbs_graph {
@@ -180,6 +204,28 @@ struct pbbs_graph {
   // levels plus the launch that found the frontier empty); 0 = none yet.
   std::atomic<int> last_launches[2] = {0, 0};
 };
+
+// One persistent launch per device at a time: its grid fills the device and
+// waits at barriers for all of its workgroups, so two such launches sharing
+// the device could each hold part of it and wait for the rest.  A run that
+// finds the token taken runs one level per launch instead.
+constexpr int kMaxDevices = 64;
+static std::atomic<int> g_persist_busy[kMaxDevices];
+
+static bool take_persist(pbbs_ctx* c) {
+  if (c->device < 0 || c->device >= kMaxDevices) return false;
+  int expect = 0;
+  if (!g_persist_busy[c->device].compare_exchange_strong(expect, 1)) return false;
+  c->run.token = true;
+  return true;
+}
+
+static void release_persist(pbbs_ctx* c) {
+  if (c->run.token) {
+    g_persist_busy[c->device].store(0);
+    c->run.token = false;
+  }
+}
 
 static void free_workspace(pbbs_ctx* c) {
   (void)hipFree(c->w_parent);
@@ -236,6 +282,8 @@ static int default_tuning(pbbs_tuning* t) {
   t->batch_pull_div = 10;
   t->bu_row_tops = 1;
   t->queue_div = 4;
+  t->persist = 1;
+  t->persist_wait_us = 200000;
   return PBBS_OK;
 }
 
@@ -270,6 +318,9 @@ int pbbs_ctx_create(int device, pbbs_ctx** out) {
     return fail(PBBS_ERR_HIP, "context setup on device %d failed", device);
   }
   c->num_cus = prop.multiProcessorCount;
+  if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
+      c->wall_khz <= 0)
+    c->wall_khz = 100000;   // gfx9 wall clock: 100 MHz
   std::memset(c->host_status, 0, 64);
   *out = c;
   return PBBS_OK;
@@ -279,6 +330,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   if (!c) return PBBS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release_persist(c);
   {
     std::lock_guard<std::mutex> lk(c->graphs_mu);
     for (pbbs_graph* g : c->graphs) g->ctx = nullptr;
@@ -300,6 +352,11 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   (void)hipFree(c->w_state);
   (void)hipFree(c->w_wg_level);
   (void)hipFree(c->w_args);
+  (void)hipFree(c->m_ws);
+  (void)hipFree(c->m_args);
+  (void)hipFree(c->m_wg);
+  if (c->m_host_status) (void)hipHostFree(c->m_host_status);
+  if (c->m_host_state) (void)hipHostFree(c->m_host_state);
   if (c->host_status) (void)hipHostFree(c->host_status);
   if (c->host_state) (void)hipHostFree(c->host_state);
   if (c->ev_start) (void)hipEventDestroy(c->ev_start);
@@ -324,11 +381,12 @@ int pbbs_tuning_set(pbbs_ctx* c, const pbbs_tuning* t) {
   if (t->td_to_bu_div < 0 || t->bu_to_td_div < 1 || t->grid_blocks < 0 ||
       t->bu_serial_depth < 0 || t->bu_serial_depth > 4096 || t->chunk_levels < 1 ||
       t->chunk_levels > 4096 || t->batch_pull_div < 0 || t->bu_row_tops < 0 ||
-      t->bu_row_tops > 1 || t->queue_div < 1)
+      t->bu_row_tops > 1 || t->queue_div < 1 || t->persist < 0 || t->persist > 1 ||
+      t->persist_wait_us < 0)
     return fail(PBBS_ERR_INVALID,
                 "tuning out of range (td_to_bu_div>=0, bu_to_td_div>=1, grid_blocks>=0, "
                 "0<=bu_serial_depth<=4096, 1<=chunk_levels<=4096, batch_pull_div>=0, "
-                "bu_row_tops 0 or 1, queue_div>=1)");
+                "bu_row_tops 0 or 1, queue_div>=1, persist 0 or 1, persist_wait_us>=0)");
   c->tuning = *t;
   return PBBS_OK;
 }
@@ -698,6 +756,7 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.symmetric = g->symmetric ? 1 : 0;
   a.num_cus = c->num_cus;
   a.small_rows = g->max_degree <= kWave ? 1 : 0;
+  a.wait_ticks = (long long)c->tuning.persist_wait_us * c->wall_khz / 1000;
   return a;
 }
 
@@ -747,8 +806,10 @@ static int launch_chunk(pbbs_ctx* c) {
   return PBBS_OK;
 }
 
+static int restart_levels(pbbs_ctx* c);
+
 // Advances the BFS in flight.  With wait, blocks for the current chunk.
-static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
+static int poll_run_impl(pbbs_ctx* c, bool wait, bool* done) {
   *done = false;
   if (wait) {
     HIP_TRY(hipEventSynchronize(c->ev_end));
@@ -776,15 +837,32 @@ static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
       HIP_TRY(hipEventSynchronize(c->ev_end));
     }
     if (r.graph) r.graph->last_launches[r.kind] = c->host_status[1] + 1;
+    release_persist(c);
+    r.persist = false;
     *done = true;
     return PBBS_OK;
   }
   pbbs_ctx::Run& r = c->run;
+  if (r.persist) {
+    release_persist(c);
+    r.persist = false;
+    if (!c->host_status[2])
+      return fail(PBBS_ERR_STATE, "persistent BFS launch ended before the BFS did");
+    // the launch gave up waiting at a barrier (the device was shared): run the
+    // BFS again from its start, one level per launch
+    return restart_levels(c);
+  }
   if (r.launched > r.limit)
     return fail(PBBS_ERR_STATE, "BFS did not finish after %lld level launches",
                 (long long)r.launched);
   r.K = next_chunk(c->tuning.chunk_levels, (int)std::min<int64_t>(r.launched, 1 << 20));
   return launch_chunk(c);
+}
+
+static int poll_run(pbbs_ctx* c, bool wait, bool* done) {
+  const int rc = poll_run_impl(c, wait, done);
+  if (rc != PBBS_OK) release_persist(c);
+  return rc;
 }
 
 // Launches in a run's first chunk: BFS depths vary little between sources of
@@ -802,16 +880,42 @@ static int level_variant(const pbbs_graph* g, const uint32_t* tops) {
   return !g->sorted ? kLevelUnsorted : tops ? kLevelSortedTops : kLevelSorted;
 }
 
+// Workgroups of the persistent kernel that fit on the device at once.
+static int persist_capacity(pbbs_ctx* c, bool off64, int variant) {
+  int per_cu = 0;
+  const void* fn =
+      off64 ? persist_kernel_symbol<int64_t>(variant) : persist_kernel_symbol<uint32_t>(variant);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kBlock, 0) != hipSuccess)
+    return 0;
+  return std::min(per_cu * c->num_cus, kMaxGrid);
+}
+
+// Starts a single-source BFS.  With allow_persist, every level runs in one
+// persistent launch when the tuning asks for it and this context can take
+// the device's token; otherwise the levels run one per launch, in captured
+// chunks.  restart: the run's first attempt gave up; keep its start event.
 template <typename OffT>
-static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth) {
+static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
+                     bool allow_persist = true, bool restart = false) {
   const uint32_t* tops = nullptr;
   int rc = ensure_tops<OffT>(c, g, &tops);
   if (rc != PBBS_OK) return rc;
   const int variant = level_variant(g, tops);
-  const int grid = level_grid(c, g->off64, variant);
+  int grid = level_grid(c, g->off64, variant);
+  bool persist = false;
+  if (allow_persist && c->tuning.persist) {
+    const int cap = persist_capacity(c, g->off64, variant);
+    if (cap >= 1 && take_persist(c)) {
+      persist = true;
+      grid = std::min(grid, cap);
+    }
+  }
   const uint32_t* bounds = nullptr;
   rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
-  if (rc != PBBS_OK) return rc;
+  if (rc != PBBS_OK) {
+    release_persist(c);
+    return rc;
+  }
   BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth, bounds, tops);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
@@ -825,7 +929,11 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.launched = 0;
   r.finalized = false;
   r.limit = g->n + 2;
-  if (c->tuning.use_graphs) {  // capture (first use only) while the stream is idle
+  r.persist = persist;
+  r.source = s;
+  r.parent = d_parent;
+  r.depth = d_depth;
+  if (c->tuning.use_graphs && !persist) {  // capture (first use only) while the stream is idle
     hipGraphExec_t exec = nullptr;
     rc = get_level_graph<OffT>(c, variant, grid, r.K, &exec);
     if (rc != PBBS_OK) return rc;
@@ -833,10 +941,24 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   volatile int32_t* hs = c->host_status;
   hs[0] = 0;
   hs[1] = 0;
-  HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  hs[2] = 0;
+  if (!restart) HIP_TRY(hipEventRecord(c->ev_start, c->stream));
   HIP_TRY(launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(c->w_args), g->d_iso, s, init_grid,
                                 c->stream));
+  if (persist) {
+    HIP_TRY(launch_bfs_persist<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), variant, grid,
+                                     c->stream));
+    r.launched = 1;
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    return PBBS_OK;
+  }
   return launch_chunk(c);
+}
+
+static int restart_levels(pbbs_ctx* c) {
+  pbbs_ctx::Run& r = c->run;
+  return r.off64 ? begin_bfs<int64_t>(c, r.graph, r.source, r.parent, r.depth, false, true)
+                 : begin_bfs<uint32_t>(c, r.graph, r.source, r.parent, r.depth, false, true);
 }
 
 static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
@@ -861,7 +983,10 @@ static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int
   int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth);
   bool done = false;
   while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
-  if (rc != PBBS_OK) return rc;
+  if (rc != PBBS_OK) {
+    release_persist(c);
+    return rc;
+  }
   return stats ? single_stats(c, stats) : PBBS_OK;
 }
 
@@ -1385,8 +1510,10 @@ int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc
   if (nsrc == 1) {
     rc = ensure_workspace(c, g->n);
     if (rc != PBBS_OK) return rc;
-    return g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth)
-                    : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth);
+    rc = g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth)
+                  : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth);
+    if (rc != PBBS_OK) release_persist(c);
+    return rc;
   }
   if (!g->symmetric) return fail(PBBS_ERR_INVALID, "batched BFS needs a symmetric graph");
   rc = ensure_batch_workspace(c, g->n, (int64_t)nsrc * g->n, true);
@@ -1415,6 +1542,274 @@ int async_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
 }
 
 }  // namespace pbbs
+
+// ------------------------------------------------------ multi-graph launch
+namespace pbbs {
+
+constexpr int64_t kJobAlign = 256;
+static int64_t job_align(int64_t x) { return (x + kJobAlign - 1) / kJobAlign * kJobAlign; }
+
+// Device memory of one single-source BFS outside the context workspace:
+// control block, per-workgroup level counters, argument block, the four
+// bitmaps, the two frontier queues and their edge prefixes (queue_cap(n,
+// queue_div) entries per sub-queue region), and optionally parent and depth.
+struct JobLayout {
+  int64_t st, wg, args, bits, q, qe, parent, depth, total, qcap;
+};
+
+static JobLayout job_layout(int64_t n, int32_t queue_div, bool outputs) {
+  JobLayout L;
+  const int64_t nwords = (n + kWave - 1) / kWave;
+  L.qcap = queue_cap(n, queue_div);
+  L.st = 0;
+  L.wg = job_align(L.st + (int64_t)sizeof(BfsState));
+  L.args = job_align(L.wg + kMaxGrid * (int64_t)sizeof(uint32_t));
+  L.bits = job_align(L.args + (int64_t)sizeof(BfsArgs<int64_t>));
+  L.q = job_align(L.bits + 4 * nwords * (int64_t)sizeof(unsigned long long));
+  L.qe = job_align(L.q + 2 * kSubQueues * L.qcap * (int64_t)sizeof(int32_t));
+  L.parent = job_align(L.qe + 2 * kSubQueues * L.qcap * (int64_t)sizeof(uint32_t));
+  L.depth = outputs ? job_align(L.parent + n * (int64_t)sizeof(int32_t)) : L.parent;
+  L.total = outputs ? job_align(L.depth + n * (int64_t)sizeof(int32_t)) : L.parent;
+  return L;
+}
+
+// BfsArgs of a BFS whose workspace is laid out by L at base.
+template <typename OffT>
+static BfsArgs<OffT> job_args(pbbs_ctx* c, pbbs_graph* g, const JobLayout& L, unsigned char* base,
+                              int32_t* parent, int32_t* depth, int32_t* host_status,
+                              BfsState* host_state) {
+  const int64_t nwords = (g->n + kWave - 1) / kWave;
+  unsigned long long* bits = reinterpret_cast<unsigned long long*>(base + L.bits);
+  BfsArgs<OffT> a;
+  std::memset(&a, 0, sizeof a);
+  a.off = static_cast<const OffT*>(g->d_off);
+  a.adj = g->d_adj;
+  a.parent = parent;
+  a.depth = depth;
+  a.visited = bits;
+  a.fb[0] = bits + nwords;
+  a.fb[1] = bits + 2 * nwords;
+  a.fb[2] = bits + 3 * nwords;
+  a.q[0] = reinterpret_cast<int32_t*>(base + L.q);
+  a.q[1] = a.q[0] + kSubQueues * L.qcap;
+  a.qe[0] = reinterpret_cast<uint32_t*>(base + L.qe);
+  a.qe[1] = a.qe[0] + kSubQueues * L.qcap;
+  a.qcap = L.qcap;
+  a.st = reinterpret_cast<BfsState*>(base + L.st);
+  a.wg_level = reinterpret_cast<uint32_t*>(base + L.wg);
+  a.host_status = host_status;
+  a.host_state = host_state;
+  a.bu_bounds = nullptr;   // equal shares: no per-size bounds are built for a job
+  a.bu_tops = nullptr;
+  a.n = g->n;
+  a.m = g->m;
+  a.nwords = nwords;
+  a.td_to_bu_div = c->tuning.td_to_bu_div;
+  a.bu_to_td_div = c->tuning.bu_to_td_div;
+  a.bu_serial_depth = c->tuning.bu_serial_depth;
+  a.symmetric = g->symmetric ? 1 : 0;
+  a.num_cus = c->num_cus;
+  a.small_rows = g->max_degree <= kWave ? 1 : 0;
+  a.wait_ticks = (long long)c->tuning.persist_wait_us * c->wall_khz / 1000;
+  return a;
+}
+
+int64_t multi_workspace_bytes(const int64_t* n, int32_t njobs, int32_t queue_div) {
+  int64_t b = 0;
+  for (int32_t j = 0; j < njobs; ++j) b += job_layout(n[j], queue_div, true).total;
+  return b;
+}
+
+// Workgroups per job: proportional to n + m (at least one each, at most one
+// per 4096 edges, so a small graph does not spread over the whole device),
+// within cap in total.
+static void split_grid(const std::vector<pbbs_graph*>& gs, int cap, std::vector<uint32_t>& wg) {
+  const size_t J = gs.size();
+  double total = 0;
+  for (pbbs_graph* g : gs) total += (double)(g->n + g->m);
+  std::vector<int64_t> w(J);
+  int64_t sum = 0;
+  for (size_t j = 0; j < J; ++j) {
+    const int64_t share = (int64_t)((double)cap * (double)(gs[j]->n + gs[j]->m) / total);
+    const int64_t most = std::max<int64_t>(1, (gs[j]->m + 4095) / 4096);
+    w[j] = std::max<int64_t>(1, std::min(share, most));
+    sum += w[j];
+  }
+  while (sum > cap) {   // the floors of one each pushed it over: trim the largest
+    size_t big = 0;
+    for (size_t j = 1; j < J; ++j)
+      if (w[j] > w[big]) big = j;
+    --w[big];
+    --sum;
+  }
+  wg.assign(J + 1, 0);
+  for (size_t j = 0; j < J; ++j) wg[j + 1] = wg[j] + (uint32_t)w[j];
+}
+
+template <typename OffT>
+static int run_multi_group(pbbs_ctx* c, const std::vector<int>& ids,
+                           const std::vector<pbbs_graph*>& graphs, const int32_t* sources,
+                           std::vector<int32_t*>& dpar, std::vector<int32_t*>& ddep,
+                           pbbs_bfs_stats* stats, bool sorted) {
+  const int J = (int)ids.size();
+  std::vector<pbbs_graph*> gs(J);
+  for (int j = 0; j < J; ++j) gs[j] = graphs[ids[j]];
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, multi_kernel_symbol<OffT>(sorted),
+                                                   kBlock, 0) != hipSuccess ||
+      per_cu < 1)
+    return fail(PBBS_ERR_HIP, "occupancy of the multi-graph kernel unavailable");
+  const int cap = std::min(per_cu * c->num_cus, kMaxGrid);
+  if (J > cap) return fail(PBBS_ERR_LIMIT, "%d jobs exceed the %d workgroups resident", J, cap);
+  std::vector<uint32_t> wg;
+  split_grid(gs, cap, wg);
+  const int32_t qdiv = c->tuning.queue_div;
+  // workspaces, one after another in m_ws (outputs included; used when the
+  // caller gave no device pointer)
+  std::vector<JobLayout> L(J);
+  std::vector<int64_t> at(J + 1, 0);
+  for (int j = 0; j < J; ++j) {
+    L[j] = job_layout(gs[j]->n, qdiv, true);
+    at[j + 1] = at[j] + L[j].total;
+  }
+  if (at[J] > c->m_ws_bytes) {
+    (void)hipFree(c->m_ws);
+    c->m_ws = nullptr;
+    c->m_ws_bytes = 0;
+    if (hipMalloc(&c->m_ws, at[J]) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PBBS_ERR_NOMEM, "multi-graph workspace of %lld bytes failed", (long long)at[J]);
+    }
+    c->m_ws_bytes = at[J];
+  }
+  BfsArgs<OffT>* dargs = static_cast<BfsArgs<OffT>*>(c->m_args);
+  for (int j = 0; j < J; ++j) {
+    unsigned char* base = c->m_ws + at[j];
+    const int i = ids[j];
+    if (!dpar[i]) dpar[i] = reinterpret_cast<int32_t*>(base + L[j].parent);
+    if (!ddep[i]) ddep[i] = reinterpret_cast<int32_t*>(base + L[j].depth);
+    int32_t* hs = c->m_host_status + 16 * j;
+    hs[0] = hs[1] = hs[2] = 0;
+    const BfsArgs<OffT> a =
+        job_args<OffT>(c, gs[j], L[j], base, dpar[i], ddep[i], hs, c->m_host_state + j);
+    const int init_grid = (int)std::max<int64_t>(
+        1, std::min<int64_t>((gs[j]->n + kBlock - 1) / kBlock, c->num_cus * 16));
+    HIP_TRY(launch_bfs_init<OffT>(a, dargs + j, gs[j]->d_iso, sources[i], init_grid, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(c->m_wg, wg.data(), (J + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                         c->stream));
+  const bool persist = take_persist(c);
+  if (persist) {
+    HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+    const hipError_t le = launch_bfs_multi<OffT>(dargs, c->m_wg, J, sorted, (int)wg[J], c->stream);
+    const hipError_t re = le == hipSuccess ? hipEventRecord(c->ev_end, c->stream) : le;
+    const hipError_t se = re == hipSuccess ? hipEventSynchronize(c->ev_end) : re;
+    release_persist(c);
+    if (se != hipSuccess)
+      return fail(PBBS_ERR_HIP, "multi-graph launch: %s", hipGetErrorString(se));
+  }
+  float ms = 0.f;
+  if (persist) (void)hipEventElapsedTime(&ms, c->ev_start, c->ev_end);
+  for (int j = 0; j < J; ++j) {
+    const int i = ids[j];
+    volatile int32_t* hs = c->m_host_status + 16 * j;
+    if (persist && hs[0]) {
+      const BfsState& h = c->m_host_state[j];
+      pbbs_bfs_stats& st = stats[i];
+      std::memset(&st, 0, sizeof st);
+      st.levels = h.max_depth + 1;
+      st.td_levels = h.td_levels;
+      st.bu_levels = h.bu_levels;
+      st.launches = 1;
+      st.reached = (int64_t)h.reached;
+      st.edges_reached = (int64_t)h.edges_reached;
+      st.device_ms = ms;
+      continue;
+    }
+    // the device was busy with another persistent launch, or this job's
+    // workgroups could not gather: run it alone
+    int rc = ensure_workspace(c, gs[j]->n);
+    if (rc == PBBS_OK)
+      rc = run_bfs<OffT>(c, gs[j], sources[i], dpar[i], ddep[i], &stats[i]);
+    if (rc != PBBS_OK) return rc;
+  }
+  return PBBS_OK;
+}
+
+}  // namespace pbbs
+
+extern "C" int pbbs_bfs_multi(pbbs_ctx* c, int32_t njobs, pbbs_graph* const* graphs,
+                              const int32_t* sources, int32_t* const* parents,
+                              int32_t* const* depths, uint32_t out_flags, pbbs_bfs_stats* stats) {
+  clear_error();
+  if (!c || !graphs || !sources || !stats) return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (njobs < 1 || njobs > kMaxMultiJobs)
+    return fail(PBBS_ERR_INVALID, "job count %d outside [1, %d]", njobs, kMaxMultiJobs);
+  for (int j = 0; j < njobs; ++j) {
+    const pbbs_graph* g = graphs[j];
+    if (!g) return fail(PBBS_ERR_INVALID, "graph %d is NULL", j);
+    if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph %d lives on another device", j);
+    if (sources[j] < 0 || (int64_t)sources[j] >= g->n)
+      return fail(PBBS_ERR_INVALID, "job %d: source %d outside [0, %lld)", j, sources[j],
+                  (long long)g->n);
+  }
+  HIP_TRY(hipSetDevice(c->device));
+  if (!c->m_args) {
+    void* args = nullptr;
+    uint32_t* wg = nullptr;
+    int32_t* hs = nullptr;
+    BfsState* hst = nullptr;
+    if (hipMalloc(&args, kMaxMultiJobs * sizeof(BfsArgs<int64_t>)) != hipSuccess ||
+        hipMalloc(&wg, (kMaxMultiJobs + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc((void**)&hs, kMaxMultiJobs * 16 * sizeof(int32_t), hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostMalloc((void**)&hst, kMaxMultiJobs * sizeof(BfsState), hipHostMallocDefault) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(args);
+      (void)hipFree(wg);
+      if (hs) (void)hipHostFree(hs);
+      if (hst) (void)hipHostFree(hst);
+      return fail(PBBS_ERR_NOMEM, "multi-graph control buffers failed");
+    }
+    c->m_wg = wg;
+    c->m_host_status = hs;
+    c->m_host_state = hst;
+    c->m_args = args;
+  }
+  const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
+  std::vector<pbbs_graph*> gs(graphs, graphs + njobs);
+  std::vector<int32_t*> dpar(njobs, nullptr), ddep(njobs, nullptr);
+  for (int j = 0; j < njobs; ++j) {
+    if (dev_out && parents) dpar[j] = parents[j];
+    if (dev_out && depths) ddep[j] = depths[j];
+  }
+  // one launch per (offset width, row order) group: jobs of a launch share
+  // the kernel instantiation
+  for (int off64 = 0; off64 < 2; ++off64) {
+    for (int sorted = 0; sorted < 2; ++sorted) {
+      std::vector<int> ids;
+      for (int j = 0; j < njobs; ++j)
+        if ((int)gs[j]->off64 == off64 && (int)gs[j]->sorted == sorted) ids.push_back(j);
+      if (ids.empty()) continue;
+      const int rc = off64 ? run_multi_group<int64_t>(c, ids, gs, sources, dpar, ddep, stats, sorted)
+                           : run_multi_group<uint32_t>(c, ids, gs, sources, dpar, ddep, stats,
+                                                       sorted);
+      if (rc != PBBS_OK) return rc;
+    }
+  }
+  if (!dev_out) {
+    for (int j = 0; j < njobs; ++j) {
+      const size_t bytes = (size_t)gs[j]->n * sizeof(int32_t);
+      if (parents && parents[j])
+        HIP_TRY(hipMemcpyAsync(parents[j], dpar[j], bytes, hipMemcpyDeviceToHost, c->stream));
+      if (depths && depths[j])
+        HIP_TRY(hipMemcpyAsync(depths[j], ddep[j], bytes, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return PBBS_OK;
+}
 
 // ------------------------------------------------------------ graph download
 extern "C" int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out) {

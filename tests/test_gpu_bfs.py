@@ -105,6 +105,13 @@ def test_permuted_rmat(ctx):
     dict(queue_div=1_000_000),
     dict(queue_div=1_000_000, td_to_bu_div=0),
     dict(queue_div=1_000_000, td_to_bu_div=0, grid_blocks=7, use_graphs=0),
+    # one level per launch instead of the persistent launch, and a persistent
+    # launch that gives up at its first wait (the BFS then runs again one
+    # level per launch)
+    dict(persist=0),
+    dict(persist_wait_us=0),
+    dict(persist_wait_us=0, grid_blocks=7),
+    dict(persist=1, grid_blocks=3, queue_div=1_000_000),
 ])
 def test_tuning_invariance(tune):
     say("tuning", tune)
