@@ -86,6 +86,11 @@ def pick_source(g, name, seed):
             return s
 
 
+def say(*a):
+    """Progress of the run on stderr (the JSON line alone goes to stdout)."""
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -175,7 +180,7 @@ def run_single(args):
 
     build, wide, desc = WORKLOADS[args.workload]
     ctx = pb.Context(0)
-    tune = {}
+    tune = dict(args.tune)
     if args.td_to_bu_div is not None:
         tune["td_to_bu_div"] = args.td_to_bu_div
     if args.bu_to_td_div is not None:
@@ -199,6 +204,7 @@ def run_single(args):
         tuning = {"table": None, "applied": False, "params": dict(tune)}
     src = pick_source(g, args.workload, args.seed)
     deg = g.degrees()
+    say(f"single: {args.workload} n={g.n} m={g.m} built in {gen_s:.1f} s, source {src}")
 
     # A cold call: the first BFS on a graph builds its search data (row-top
     # records, share bounds) and captures the launch graphs.  Then a second
@@ -231,6 +237,7 @@ def run_single(args):
     # Device time depends on the source: a source whose big level is reached
     # through a frontier of a few low-id hubs costs more than the bench source.
     # Other seeded sources are timed after the timed region.
+    say(f"single: {args.steps} timed steps, {ms_per_step:.3f} ms per step (wall)")
     spread = None
     if args.spread > 0:
         rng = np.random.default_rng(args.seed + 1000)
@@ -255,6 +262,7 @@ def run_single(args):
         }
 
     # outputs to host once (D2H reported, not timed) and bit-exact check
+    say("single: source spread done" if spread else "single: no source spread")
     parent, depth, st_host = dg.bfs(src)
     result = {
         "metric": METRIC,
@@ -292,7 +300,10 @@ def run_single(args):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "one BFS launch sequence (bfs_init_kernel + bfs_level_kernel per level)",
+            "kernel": ("one BFS: bfs_init_kernel + one bfs_persist_kernel launch running every "
+                       "level" if st["launches"] == 1 else
+                       "one BFS launch sequence (bfs_init_kernel + bfs_level_kernel per level)"),
+            "launches_per_bfs": st["launches"],
             "achieved": round(B / (event_ms / 1e3) / 1e9, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -303,6 +314,7 @@ def run_single(args):
     }
     exact = True
     if args.cpu_seconds > 0:
+        say(f"single: CPU baseline (budget {args.cpu_seconds} s after a 1 s warm-up)")
         rec, ref = cpu_baseline(g, [src], args.cpu_seconds,
                                 f"full BFS runs from the bench source on {args.workload}")
         exact = bool(np.array_equal(ref, parent))
@@ -344,6 +356,7 @@ def run_batch(args, nested):
         dg.bfs_batch_device(src)
         for s in src:
             dg.bfs_device(int(s))
+    say(f"batched: rmat22 n={g.n} m={g.m}, preprocessing {prep_ms:.1f} ms, warm-up done")
     # sequential: 64 single-source runs, outputs in HBM
     seq_ms, seq_edges = 0.0, 0
     t0 = time.perf_counter()
@@ -361,6 +374,7 @@ def run_batch(args, nested):
     wall = (time.perf_counter() - t0) / args.steps * 1e3
     pairs_edges = st["edges_reached"]
     exact = pairs_edges == seq_edges
+    say(f"batched: {wall:.3f} ms per batch (wall), 64 sequential runs {seq_wall:.3f} ms")
     # parity: every batched row equals its sequential run, bit for bit, and a
     # seeded sample of rows equals the oracle
     P, D, _ = dg.bfs_batch(src)
@@ -469,6 +483,7 @@ def run_serve(args, nested):
     ids = [srv.add_graph(g) for g in graphs]
     srv.drain([srv.submit(ids[i], int(nz[i][0]), depth=False) for i in range(G)])
 
+    say(f"serving: {G} graphs resident")
     # burst: peak throughput
     bsel, bsrc = draw(burst)
     bufs = [take(gi) for gi in bsel]
@@ -481,6 +496,7 @@ def run_serve(args, nested):
         give(bsel[q], bufs[q])
     qps = args.qps if args.qps > 0 else 0.5 * peak
 
+    say(f"serving: burst of {burst} at {peak:.0f} queries/s; Poisson stream of {Q} next")
     # Poisson arrivals
     gsel, srcs = draw(Q)
     gaps = rng.exponential(1.0 / qps, Q)
@@ -669,7 +685,15 @@ def main():
                          "the timed region is over and report their spread (0 = skip)")
     ap.add_argument("--host-gen", action="store_true",
                     help="build the graphs with the host generators (default: on the GPU)")
+    ap.add_argument("--tune", action="append", default=[], metavar="FIELD=VALUE",
+                    help="single: set a pbbs_tuning field (e.g. persist=0); repeatable; "
+                         "replaces the tuning table")
     args = ap.parse_args()
+    tune = []
+    for kv in args.tune:
+        k, _, v = kv.partition("=")
+        tune.append((k.strip(), int(v)))
+    args.tune = tune
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); this build has no "
               "multi-GPU path: batched and serving work shard by replicating the graph per GPU, "

@@ -1649,6 +1649,7 @@ static void split_grid(const std::vector<pbbs_graph*>& gs, int cap, std::vector<
 template <typename OffT>
 static int run_multi_group(pbbs_ctx* c, const std::vector<int>& ids,
                            const std::vector<pbbs_graph*>& graphs, const int32_t* sources,
+                           const std::vector<JobLayout>& L, const std::vector<int64_t>& at,
                            std::vector<int32_t*>& dpar, std::vector<int32_t*>& ddep,
                            pbbs_bfs_stats* stats, bool sorted) {
   const int J = (int)ids.size();
@@ -1663,35 +1664,14 @@ static int run_multi_group(pbbs_ctx* c, const std::vector<int>& ids,
   if (J > cap) return fail(PBBS_ERR_LIMIT, "%d jobs exceed the %d workgroups resident", J, cap);
   std::vector<uint32_t> wg;
   split_grid(gs, cap, wg);
-  const int32_t qdiv = c->tuning.queue_div;
-  // workspaces, one after another in m_ws (outputs included; used when the
-  // caller gave no device pointer)
-  std::vector<JobLayout> L(J);
-  std::vector<int64_t> at(J + 1, 0);
-  for (int j = 0; j < J; ++j) {
-    L[j] = job_layout(gs[j]->n, qdiv, true);
-    at[j + 1] = at[j] + L[j].total;
-  }
-  if (at[J] > c->m_ws_bytes) {
-    (void)hipFree(c->m_ws);
-    c->m_ws = nullptr;
-    c->m_ws_bytes = 0;
-    if (hipMalloc(&c->m_ws, at[J]) != hipSuccess) {
-      (void)hipGetLastError();
-      return fail(PBBS_ERR_NOMEM, "multi-graph workspace of %lld bytes failed", (long long)at[J]);
-    }
-    c->m_ws_bytes = at[J];
-  }
   BfsArgs<OffT>* dargs = static_cast<BfsArgs<OffT>*>(c->m_args);
   for (int j = 0; j < J; ++j) {
-    unsigned char* base = c->m_ws + at[j];
     const int i = ids[j];
-    if (!dpar[i]) dpar[i] = reinterpret_cast<int32_t*>(base + L[j].parent);
-    if (!ddep[i]) ddep[i] = reinterpret_cast<int32_t*>(base + L[j].depth);
+    unsigned char* base = c->m_ws + at[i];
     int32_t* hs = c->m_host_status + 16 * j;
     hs[0] = hs[1] = hs[2] = 0;
     const BfsArgs<OffT> a =
-        job_args<OffT>(c, gs[j], L[j], base, dpar[i], ddep[i], hs, c->m_host_state + j);
+        job_args<OffT>(c, gs[j], L[i], base, dpar[i], ddep[i], hs, c->m_host_state + j);
     const int init_grid = (int)std::max<int64_t>(
         1, std::min<int64_t>((gs[j]->n + kBlock - 1) / kBlock, c->num_cus * 16));
     HIP_TRY(launch_bfs_init<OffT>(a, dargs + j, gs[j]->d_iso, sources[i], init_grid, c->stream));
@@ -1779,10 +1759,35 @@ extern "C" int pbbs_bfs_multi(pbbs_ctx* c, int32_t njobs, pbbs_graph* const* gra
   }
   const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
   std::vector<pbbs_graph*> gs(graphs, graphs + njobs);
+  // every job's workspace, one after another in m_ws, laid out before any
+  // launch: the groups below run one after another and each job's outputs
+  // (when the caller gave no device pointer) stay in its own region until
+  // they are copied out at the end
+  std::vector<JobLayout> L(njobs);
+  std::vector<int64_t> at(njobs + 1, 0);
+  for (int j = 0; j < njobs; ++j) {
+    L[j] = job_layout(gs[j]->n, c->tuning.queue_div, true);
+    at[j + 1] = at[j] + L[j].total;
+  }
+  if (at[njobs] > c->m_ws_bytes) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    (void)hipFree(c->m_ws);
+    c->m_ws = nullptr;
+    c->m_ws_bytes = 0;
+    if (hipMalloc(&c->m_ws, at[njobs]) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(PBBS_ERR_NOMEM, "multi-graph workspace of %lld bytes failed",
+                  (long long)at[njobs]);
+    }
+    c->m_ws_bytes = at[njobs];
+  }
   std::vector<int32_t*> dpar(njobs, nullptr), ddep(njobs, nullptr);
   for (int j = 0; j < njobs; ++j) {
-    if (dev_out && parents) dpar[j] = parents[j];
-    if (dev_out && depths) ddep[j] = depths[j];
+    unsigned char* base = c->m_ws + at[j];
+    dpar[j] = (dev_out && parents && parents[j]) ? parents[j]
+                                                 : reinterpret_cast<int32_t*>(base + L[j].parent);
+    ddep[j] = (dev_out && depths && depths[j]) ? depths[j]
+                                               : reinterpret_cast<int32_t*>(base + L[j].depth);
   }
   // one launch per (offset width, row order) group: jobs of a launch share
   // the kernel instantiation
@@ -1792,9 +1797,9 @@ extern "C" int pbbs_bfs_multi(pbbs_ctx* c, int32_t njobs, pbbs_graph* const* gra
       for (int j = 0; j < njobs; ++j)
         if ((int)gs[j]->off64 == off64 && (int)gs[j]->sorted == sorted) ids.push_back(j);
       if (ids.empty()) continue;
-      const int rc = off64 ? run_multi_group<int64_t>(c, ids, gs, sources, dpar, ddep, stats, sorted)
-                           : run_multi_group<uint32_t>(c, ids, gs, sources, dpar, ddep, stats,
-                                                       sorted);
+      const int rc =
+          off64 ? run_multi_group<int64_t>(c, ids, gs, sources, L, at, dpar, ddep, stats, sorted)
+                : run_multi_group<uint32_t>(c, ids, gs, sources, L, at, dpar, ddep, stats, sorted);
       if (rc != PBBS_OK) return rc;
     }
   }
