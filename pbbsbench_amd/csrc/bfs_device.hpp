@@ -53,10 +53,11 @@ struct BfsState {
   int32_t pad;
   unsigned long long reached;        // n_r
   unsigned long long edges_reached;  // m_r
-  // grid barrier of a persistent launch: workgroup b arrives on
-  // bar_grp[b % kBarGroups]; a group's last arrival arrives on bar_top, whose
-  // last arrival bumps bar_gen.  abort is set by a workgroup whose wait ran
-  // out; every workgroup then leaves the launch.
+  // grid barrier of a persistent launch (grid_barrier in bfs_kernels.hip):
+  // workgroup b arrives on bar_grp[b % kBarGroups]; a group's last arrival
+  // arrives on bar_top, whose last arrival bumps bar_gen (the barriers
+  // completed).  abort is set by a workgroup whose wait ran out; every
+  // workgroup then leaves the launch.
   LineCounter bar_grp[kBarGroups];
   LineCounter bar_top;
   LineCounter bar_gen;

@@ -244,6 +244,7 @@ struct LevelView {
   gptr<const u32x4> tops;               // per vertex {six largest neighbours, row end, degree}, or null
   int small_rows;                       // every row has at most 64 entries
   long long n;
+  long long m;
   long long nwords;
   long long qcap;
   long long gw;                         // this wavefront's index in the level's grid
@@ -395,7 +396,8 @@ __device__ __forceinline__ void td_window(const LevelView<OffT>& a, StageBuf& sb
     int v = 0;
     uint32_t deg = 0;
     if (valid) {
-      v = a.adj[rsj + (OffT)(my - qoj)];
+      const OffT ei = rsj + (OffT)(my - qoj);
+      v = a.adj[(unsigned long long)ei < (unsigned long long)a.m ? ei : (OffT)0];
       // the row bounds load with the depth (one round trip, not two after a claim)
       const int dv = a.depth[v];
       const OffT o0 = a.off[v], o1 = a.off[v + 1];
@@ -450,6 +452,9 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     OffT rs = 0;
     if (idx < nf) {
       qv = q[idx];
+      // entries are vertex ids written by the level before; the clamp keeps
+      // every later index in bounds whatever the queue holds
+      qv = (unsigned)qv < (unsigned long long)a.n ? qv : 0;
       qo = qe[idx];
       rs = a.off[qv];
     }
@@ -1065,6 +1070,7 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
   a.tops = glob(kTops ? reinterpret_cast<const u32x4*>(A->bu_tops) : nullptr);
   a.small_rows = A->small_rows;
   a.n = A->n;
+  a.m = A->m;
   a.nwords = A->nwords;
   a.gw = (long long)vb * kWavesPerBlock + threadIdx.x / kWave;
   a.W = (long long)vg * kWavesPerBlock;
@@ -1101,59 +1107,93 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kerne
     A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
 }
 
-// Grid barrier between the levels of a persistent launch.  Every wavefront
-// first waits for its own stores; then one thread per workgroup writes the
-// L2 back (release), arrives, waits for the generation to move, and
-// invalidates stale lines (acquire), the cache work a kernel boundary does.
-// The wait is bounded by A->wait_ticks of the device wall clock: a workgroup
-// that runs out sets st->abort (and host_status[2] for the host), and every
-// workgroup leaves at its next barrier or level start, so the launch always
-// ends; the host then runs the BFS again one level per launch.  Returns
-// false when the launch is to end.  vb / vg: this workgroup's index in the
-// grid that meets at the barrier, and that grid's size.
-__device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_ticks,
-                             const unsigned vb, const unsigned vg) {
-  __shared__ int ok;
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
+// Grid barrier between the levels of a persistent launch.  It does what a
+// kernel boundary does for the data the levels share, by the agent-scope
+// release/acquire rules of the memory model: each workgroup, once its
+// stores are in, makes them visible device-wide (release: L2 write-back),
+// arrives, waits for the barrier to complete, then drops stale cached
+// lines (acquire: L1 and L2 invalidate) before it reads again.  The wait
+// itself polls the generation word with device-scope loads and no cache
+// maintenance (an acquire inside the loop would invalidate the L2 on every
+// poll).  Arrivals go to kBarGroups first-tier counters, whose last
+// arrivals arrive on bar_top; the last of those bumps bar_gen.
+//
+// Every wait is bounded by A->wait_ticks of the device wall clock: a
+// workgroup that runs out sets st->abort (and host_status[2] for the host),
+// and every workgroup leaves at its next wait or level start, so the launch
+// always ends; the host then runs the BFS again one level per launch.
+__device__ __forceinline__ unsigned ld_dev(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_dev(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned add_dev(unsigned* p, unsigned v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until *w >= target (wrap-safe); false if the launch aborts or the
+// wait runs out (which aborts it).
+__device__ bool bar_wait(const unsigned* w, unsigned target, BfsState* st, int32_t* host_status,
+                         long long wait_ticks) {
+  const long long t0 = (long long)wall_clock64();
+  while ((int)(ld_dev(w) - target) < 0) {
+    if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+    if ((long long)wall_clock64() - t0 > wait_ticks) {
+      __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&host_status[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+// A workgroup's view of the barrier: its index in the grid that meets at
+// it, that grid's size, the barriers passed so far and the last result.
+struct BarView {
+  unsigned vb, vg, nb;
+  int ok;
+};
+
+__device__ __forceinline__ void barrier_join(unsigned vb, unsigned vg, BarView& b) {
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    const unsigned G = vg < (unsigned)kBarGroups ? vg : (unsigned)kBarGroups;
-    const unsigned grp = vb % G;
-    const unsigned gsize = (vg - grp + G - 1) / G;
-    const unsigned gen =
-        __hip_atomic_load(&st->bar_gen.v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (atomicAdd(&st->bar_grp[grp].v, 1u) == gsize - 1) {
-      // last of its group: reset the group counter before the group can
-      // arrive again (that needs the generation to move, which needs this)
-      __hip_atomic_store(&st->bar_grp[grp].v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
-      if (atomicAdd(&st->bar_top.v, 1u) == G - 1) {
-        __hip_atomic_store(&st->bar_top.v, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        __hip_atomic_fetch_add(&st->bar_gen.v, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
-    int good = 1;
-    const long long t0 = (long long)wall_clock64();
-    while (__hip_atomic_load(&st->bar_gen.v, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-      if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        good = 0;
-        break;
-      }
-      if ((long long)wall_clock64() - t0 > wait_ticks) {
-        __hip_atomic_store(&st->abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&host_status[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        good = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    ok = good;
+    b.vb = vb;
+    b.vg = vg;
+    b.nb = 0;
+    b.ok = 1;
   }
   __syncthreads();
-  return ok != 0;
+}
+
+// Returns false when the launch is to end.
+__device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_ticks,
+                             BarView& b) {
+  __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores are in
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned nb = ++b.nb;
+    const unsigned G = b.vg < (unsigned)kBarGroups ? b.vg : (unsigned)kBarGroups;
+    const unsigned grp = b.vb % G;
+    const unsigned gsize = (b.vg - grp + G - 1) / G;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the block's stores, device-wide
+    if (add_dev(&st->bar_grp[grp].v, 1u) == gsize - 1) {
+      // last of its group: reset the group counter before the group can
+      // arrive again (that needs bar_gen to move, which needs this arrival)
+      st_dev(&st->bar_grp[grp].v, 0u);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");   // the reset lands first
+      if (add_dev(&st->bar_top.v, 1u) == G - 1) {
+        st_dev(&st->bar_top.v, 0u);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        add_dev(&st->bar_gen.v, 1u);
+      }
+    }
+    const bool ok = bar_wait(&st->bar_gen.v, nb, st, host_status, wait_ticks);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");   // drop lines others have written
+    b.ok = ok ? 1 : 0;
+  }
+  __syncthreads();
+  return b.ok != 0;
 }
 
 // Every level of the BFS in one launch, separated by grid barriers.  The
@@ -1166,16 +1206,20 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_persist_ker
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
   __shared__ int stop;
+  __shared__ BarView bv;
   BfsState* st = A->st;
   int k = (int)A->wg_level[blockIdx.x];
-  for (;;) {
-    if (threadIdx.x == 0)
-      stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (stop) break;
-    if (!level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x)) break;
-    ++k;
-    if (!grid_barrier(st, A->host_status, A->wait_ticks, blockIdx.x, gridDim.x)) break;
+  barrier_join(blockIdx.x, gridDim.x, bv);
+  {
+    for (;;) {
+      if (threadIdx.x == 0)
+        stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      if (stop) break;
+      if (!level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x)) break;
+      ++k;
+      if (!grid_barrier(st, A->host_status, A->wait_ticks, bv)) break;
+    }
   }
   if (threadIdx.x == 0) A->wg_level[blockIdx.x] = (uint32_t)k;
 }
@@ -1194,6 +1238,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_multi_kerne
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
   __shared__ int stop;
+  __shared__ BarView bv;
   int lo = 0, hi = njobs - 1;   // the last job with wg_begin <= blockIdx.x
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -1204,14 +1249,17 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_multi_kerne
   const BfsArgs<OffT>* __restrict__ A = args + lo;
   BfsState* st = A->st;
   int k = (int)A->wg_level[vb];
-  for (;;) {
-    if (threadIdx.x == 0)
-      stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    if (stop) break;
-    if (!level_step<OffT, kSorted, false>(A, k, sb, bl, vb, vg)) break;
-    ++k;
-    if (!grid_barrier(st, A->host_status, A->wait_ticks, vb, vg)) break;
+  barrier_join(vb, vg, bv);
+  {
+    for (;;) {
+      if (threadIdx.x == 0)
+        stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __syncthreads();
+      if (stop) break;
+      if (!level_step<OffT, kSorted, false>(A, k, sb, bl, vb, vg)) break;
+      ++k;
+      if (!grid_barrier(st, A->host_status, A->wait_ticks, bv)) break;
+    }
   }
   if (threadIdx.x == 0) A->wg_level[vb] = (uint32_t)k;
 }

@@ -107,6 +107,21 @@ using namespace pbbs;
   } while (0)
 
 // ------------------------------------------------------------------ context
+// Working memory of one single-source BFS (outputs aside): the four bitmaps
+// (visited, F_k mod 3), the two frontier queues and their edge prefixes
+// (kSubQueues regions of qcap entries each), the control block, the
+// per-workgroup level counters and the device copy of the argument block.
+// A context's own buffers, or a block the caller hands in.
+struct SingleWs {
+  unsigned long long* bits = nullptr;
+  int32_t* q = nullptr;
+  uint32_t* qe = nullptr;
+  int64_t qcap = 0;
+  BfsState* st = nullptr;
+  uint32_t* wg = nullptr;
+  void* args = nullptr;
+};
+
 struct pbbs_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -148,6 +163,9 @@ struct pbbs_ctx {
     int32_t* parent = nullptr;
     int32_t* depth = nullptr;
     pbbs_graph* graph = nullptr;  // learns the launch count when the run is done
+    SingleWs ws;                  // single source: the run's working memory
+    bool graphs = true;           // single source: captured launch graphs allowed (the
+                                  // context's own workspace only; capture allocates)
   } run;
   // batched BFS
   int64_t cap_bn = 0;                    // vertices the mask buffers hold
@@ -282,7 +300,7 @@ static int default_tuning(pbbs_tuning* t) {
   t->batch_pull_div = 10;
   t->bu_row_tops = 1;
   t->queue_div = 4;
-  t->persist = 1;
+  t->persist = 0;
   t->persist_wait_us = 200000;
   return PBBS_OK;
 }
@@ -606,13 +624,18 @@ static int next_chunk(int base, int launched) {
 }
 
 // Cuts the weighted bottom-up shares for `waves` wavefronts (cached per graph).
-static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32_t** out) {
+// With build false, bounds not built yet are an error (no allocation).
+static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32_t** out,
+                         bool build = true) {
   std::lock_guard<std::mutex> lk(g->meta_mu);
   auto it = g->bounds.find(waves);
   if (it != g->bounds.end()) {
     *out = it->second;
     return PBBS_OK;
   }
+  if (!build)
+    return fail(PBBS_ERR_STATE, "graph not prepared for %lld wavefronts (pbbs_graph_prepare "
+                "with PBBS_PREPARE_SINGLE on this context first)", (long long)waves);
   const int64_t nwords = (g->n + kWave - 1) / kWave;
   uint32_t* d = nullptr;
   HIP_TRY(hipMalloc(&d, (waves + 1) * sizeof(uint32_t)));
@@ -631,12 +654,16 @@ static int ensure_bounds(pbbs_ctx* c, pbbs_graph* g, int64_t waves, const uint32
 // Builds the row-top records of a sorted, symmetric graph on first use
 // (bottom-up levels read them in place of offsets and row tops); null when
 // the graph never runs bottom-up or the tuning turns them off.
+// With build false, records not built yet are an error (no allocation).
 template <typename OffT>
-static int ensure_tops(pbbs_ctx* c, pbbs_graph* g, const uint32_t** out) {
+static int ensure_tops(pbbs_ctx* c, pbbs_graph* g, const uint32_t** out, bool build = true) {
   *out = nullptr;
   if (!g->sorted || !g->symmetric || c->tuning.bu_row_tops == 0 || c->tuning.td_to_bu_div <= 0)
     return PBBS_OK;
   std::lock_guard<std::mutex> lk(g->meta_mu);
+  if (!g->d_tops && !build)
+    return fail(PBBS_ERR_STATE, "graph has no row-top records (pbbs_graph_prepare with "
+                "PBBS_PREPARE_SINGLE first)");
   if (!g->d_tops) {
     uint32_t* d = nullptr;
     const size_t bytes = (size_t)g->n * 32;
@@ -721,9 +748,23 @@ static int ensure_ranks(pbbs_ctx* c, pbbs_graph* g) {
   return PBBS_OK;
 }
 
+// The context's own single-source workspace, laid out for an n-vertex graph
+// (ensure_workspace(c, n) has run).
+static SingleWs own_ws(pbbs_ctx* c, int64_t n) {
+  SingleWs w;
+  w.bits = c->w_bits;
+  w.q = c->w_q;
+  w.qe = c->w_qe;
+  w.qcap = queue_cap(n, c->tuning.queue_div);   // <= c->cap_q
+  w.st = c->w_state;
+  w.wg = c->w_wg_level;
+  w.args = c->w_args;
+  return w;
+}
+
 template <typename OffT>
-static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, int32_t* d_depth,
-                               const uint32_t* bounds, const uint32_t* tops) {
+static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, const SingleWs& ws, int32_t* d_parent,
+                               int32_t* d_depth, const uint32_t* bounds, const uint32_t* tops) {
   const int64_t nwords = (g->n + kWave - 1) / kWave;
   BfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
@@ -731,18 +772,17 @@ static BfsArgs<OffT> make_args(pbbs_ctx* c, pbbs_graph* g, int32_t* d_parent, in
   a.adj = g->d_adj;
   a.parent = d_parent;
   a.depth = d_depth;
-  a.visited = c->w_bits;
-  a.fb[0] = c->w_bits + nwords;
-  a.fb[1] = c->w_bits + 2 * nwords;
-  a.fb[2] = c->w_bits + 3 * nwords;
-  const int64_t qcap = queue_cap(g->n, c->tuning.queue_div);   // <= c->cap_q
-  a.q[0] = c->w_q;
-  a.q[1] = c->w_q + kSubQueues * qcap;
-  a.qe[0] = c->w_qe;
-  a.qe[1] = c->w_qe + kSubQueues * qcap;
-  a.qcap = qcap;
-  a.st = c->w_state;
-  a.wg_level = c->w_wg_level;
+  a.visited = ws.bits;
+  a.fb[0] = ws.bits + nwords;
+  a.fb[1] = ws.bits + 2 * nwords;
+  a.fb[2] = ws.bits + 3 * nwords;
+  a.q[0] = ws.q;
+  a.q[1] = ws.q + kSubQueues * ws.qcap;
+  a.qe[0] = ws.qe;
+  a.qe[1] = ws.qe + kSubQueues * ws.qcap;
+  a.qcap = ws.qcap;
+  a.st = ws.st;
+  a.wg_level = ws.wg;
   a.host_status = c->host_status;
   a.host_state = c->host_state;
   a.bu_bounds = bounds;
@@ -771,7 +811,7 @@ static int get_batch_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphEx
 
 static int launch_chunk(pbbs_ctx* c) {
   pbbs_ctx::Run& r = c->run;
-  if (c->tuning.use_graphs) {
+  if (c->tuning.use_graphs && (r.kind == 1 || r.graphs)) {
     hipGraphExec_t exec = nullptr;
     int rc;
     if (r.kind == 0)
@@ -786,10 +826,10 @@ static int launch_chunk(pbbs_ctx* c) {
     for (int i = 0; i < r.K; ++i) {
       if (r.kind == 0) {
         if (r.off64)
-          HIP_TRY(launch_bfs_level<int64_t>(static_cast<const BfsArgs<int64_t>*>(c->w_args),
+          HIP_TRY(launch_bfs_level<int64_t>(static_cast<const BfsArgs<int64_t>*>(r.ws.args),
                                             r.sorted, r.grid, c->stream));
         else
-          HIP_TRY(launch_bfs_level<uint32_t>(static_cast<const BfsArgs<uint32_t>*>(c->w_args),
+          HIP_TRY(launch_bfs_level<uint32_t>(static_cast<const BfsArgs<uint32_t>*>(r.ws.args),
                                              r.sorted, r.grid, c->stream));
       } else {
         if (r.off64)
@@ -894,11 +934,16 @@ static int persist_capacity(pbbs_ctx* c, bool off64, int variant) {
 // persistent launch when the tuning asks for it and this context can take
 // the device's token; otherwise the levels run one per launch, in captured
 // chunks.  restart: the run's first attempt gave up; keep its start event.
+// ws: the run's working memory; own: it is the context's own workspace (so
+// captured launch graphs may be used, and per-graph search data built on
+// first use); otherwise nothing is allocated: the graph must be prepared and
+// the levels run in one persistent launch or as plain launches.
 template <typename OffT>
 static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
-                     bool allow_persist = true, bool restart = false) {
+                     const SingleWs& ws, bool own, bool allow_persist = true,
+                     bool restart = false) {
   const uint32_t* tops = nullptr;
-  int rc = ensure_tops<OffT>(c, g, &tops);
+  int rc = ensure_tops<OffT>(c, g, &tops, own);
   if (rc != PBBS_OK) return rc;
   const int variant = level_variant(g, tops);
   int grid = level_grid(c, g->off64, variant);
@@ -911,12 +956,12 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
     }
   }
   const uint32_t* bounds = nullptr;
-  rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
+  rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds, own);
   if (rc != PBBS_OK) {
     release_persist(c);
     return rc;
   }
-  BfsArgs<OffT> a = make_args<OffT>(c, g, d_parent, d_depth, bounds, tops);
+  BfsArgs<OffT> a = make_args<OffT>(c, g, ws, d_parent, d_depth, bounds, tops);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   pbbs_ctx::Run& r = c->run;
@@ -933,7 +978,9 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.source = s;
   r.parent = d_parent;
   r.depth = d_depth;
-  if (c->tuning.use_graphs && !persist) {  // capture (first use only) while the stream is idle
+  r.ws = ws;
+  r.graphs = own;
+  if (c->tuning.use_graphs && own && !persist) {  // capture (first use only) while the stream is idle
     hipGraphExec_t exec = nullptr;
     rc = get_level_graph<OffT>(c, variant, grid, r.K, &exec);
     if (rc != PBBS_OK) return rc;
@@ -943,10 +990,10 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   hs[1] = 0;
   hs[2] = 0;
   if (!restart) HIP_TRY(hipEventRecord(c->ev_start, c->stream));
-  HIP_TRY(launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(c->w_args), g->d_iso, s, init_grid,
+  HIP_TRY(launch_bfs_init<OffT>(a, static_cast<BfsArgs<OffT>*>(ws.args), g->d_iso, s, init_grid,
                                 c->stream));
   if (persist) {
-    HIP_TRY(launch_bfs_persist<OffT>(static_cast<const BfsArgs<OffT>*>(c->w_args), variant, grid,
+    HIP_TRY(launch_bfs_persist<OffT>(static_cast<const BfsArgs<OffT>*>(ws.args), variant, grid,
                                      c->stream));
     r.launched = 1;
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
@@ -957,8 +1004,11 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
 
 static int restart_levels(pbbs_ctx* c) {
   pbbs_ctx::Run& r = c->run;
-  return r.off64 ? begin_bfs<int64_t>(c, r.graph, r.source, r.parent, r.depth, false, true)
-                 : begin_bfs<uint32_t>(c, r.graph, r.source, r.parent, r.depth, false, true);
+  const SingleWs ws = r.ws;
+  const bool own = r.graphs;
+  return r.off64
+             ? begin_bfs<int64_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false, true)
+             : begin_bfs<uint32_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false, true);
 }
 
 static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
@@ -980,7 +1030,7 @@ static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
 template <typename OffT>
 static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
                    pbbs_bfs_stats* stats) {
-  int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth);
+  int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth, own_ws(c, g->n), true);
   bool done = false;
   while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
   if (rc != PBBS_OK) {
@@ -1035,7 +1085,7 @@ static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile
   const uint32_t* bounds = nullptr;
   brc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
   if (brc != PBBS_OK) return brc;
-  BfsArgs<OffT> a = make_args<OffT>(c, g, c->w_parent, c->w_depth, bounds, tops);
+  BfsArgs<OffT> a = make_args<OffT>(c, g, own_ws(c, g->n), c->w_parent, c->w_depth, bounds, tops);
   const int init_grid =
       (int)std::max<int64_t>(1, std::min<int64_t>((g->n + kBlock - 1) / kBlock, c->num_cus * 16));
   volatile int32_t* hs = c->host_status;
@@ -1130,10 +1180,17 @@ static int prepare_graph(pbbs_ctx* c, pbbs_graph* g, uint32_t what) {
     const uint32_t* tops = nullptr;
     int rc = ensure_tops<OffT>(c, g, &tops);
     if (rc != PBBS_OK) return rc;
-    const int grid = level_grid(c, g->off64, level_variant(g, tops));
+    const int variant = level_variant(g, tops);
+    const int grid = level_grid(c, g->off64, variant);
     const uint32_t* bounds = nullptr;
     rc = ensure_bounds(c, g, (int64_t)grid * kWavesPerBlock, &bounds);
     if (rc != PBBS_OK) return rc;
+    // a persistent launch runs on at most the co-resident workgroups
+    const int cap = persist_capacity(c, g->off64, variant);
+    if (cap >= 1 && cap < grid) {
+      rc = ensure_bounds(c, g, (int64_t)cap * kWavesPerBlock, &bounds);
+      if (rc != PBBS_OK) return rc;
+    }
   }
   if ((what & PBBS_PREPARE_BATCH) && g->symmetric) {
     int rc = ensure_ranks<OffT>(c, g);
@@ -1510,8 +1567,9 @@ int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc
   if (nsrc == 1) {
     rc = ensure_workspace(c, g->n);
     if (rc != PBBS_OK) return rc;
-    rc = g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth)
-                  : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth);
+    const SingleWs ws = own_ws(c, g->n);
+    rc = g->off64 ? begin_bfs<int64_t>(c, g, sources[0], c->w_parent, c->w_depth, ws, true)
+                  : begin_bfs<uint32_t>(c, g, sources[0], c->w_parent, c->w_depth, ws, true);
     if (rc != PBBS_OK) release_persist(c);
     return rc;
   }
@@ -1557,10 +1615,10 @@ struct JobLayout {
   int64_t st, wg, args, bits, q, qe, parent, depth, total, qcap;
 };
 
-static JobLayout job_layout(int64_t n, int32_t queue_div, bool outputs) {
+static JobLayout job_layout_q(int64_t n, int64_t qcap, bool outputs) {
   JobLayout L;
   const int64_t nwords = (n + kWave - 1) / kWave;
-  L.qcap = queue_cap(n, queue_div);
+  L.qcap = qcap;
   L.st = 0;
   L.wg = job_align(L.st + (int64_t)sizeof(BfsState));
   L.args = job_align(L.wg + kMaxGrid * (int64_t)sizeof(uint32_t));
@@ -1571,6 +1629,10 @@ static JobLayout job_layout(int64_t n, int32_t queue_div, bool outputs) {
   L.depth = outputs ? job_align(L.parent + n * (int64_t)sizeof(int32_t)) : L.parent;
   L.total = outputs ? job_align(L.depth + n * (int64_t)sizeof(int32_t)) : L.parent;
   return L;
+}
+
+static JobLayout job_layout(int64_t n, int32_t queue_div, bool outputs) {
+  return job_layout_q(n, queue_cap(n, queue_div), outputs);
 }
 
 // BfsArgs of a BFS whose workspace is laid out by L at base.

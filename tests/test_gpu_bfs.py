@@ -105,13 +105,16 @@ def test_permuted_rmat(ctx):
     dict(queue_div=1_000_000),
     dict(queue_div=1_000_000, td_to_bu_div=0),
     dict(queue_div=1_000_000, td_to_bu_div=0, grid_blocks=7, use_graphs=0),
-    # one level per launch instead of the persistent launch, and a persistent
-    # launch that gives up at its first wait (the BFS then runs again one
-    # level per launch)
-    dict(persist=0),
-    dict(persist_wait_us=0),
-    dict(persist_wait_us=0, grid_blocks=7),
+    # every level in one persistent launch (grid barrier between levels), on
+    # grids of 1, 3 and 7 workgroups and the default; and a persistent launch
+    # that gives up at its first wait (the BFS then runs again one level per
+    # launch)
+    dict(persist=1),
+    dict(persist=1, grid_blocks=1),
     dict(persist=1, grid_blocks=3, queue_div=1_000_000),
+    dict(persist=1, grid_blocks=7, td_to_bu_div=0),
+    dict(persist=1, persist_wait_us=0),
+    dict(persist=1, persist_wait_us=0, grid_blocks=7),
 ])
 def test_tuning_invariance(tune):
     say("tuning", tune)

@@ -82,3 +82,36 @@ def test_c2_randlocal10m(ctx):
         checked(dg, ctx, 0, st)
     finally:
         dg.free()
+
+
+@pytest.mark.parametrize("grid", [0, 256])
+def test_persistent_launch_full_size(grid):
+    """Every level in one persistent launch (grid barrier between levels) on
+    the C3 torus and the C4 RMAT graph: the torus equals its closed form, the
+    RMAT outputs pass the checker and equal the one-level-per-launch run."""
+    with pb.Context(0) as c:
+        c.set_tuning(persist=1, grid_blocks=grid)
+        say(f"persistent, grid {grid or 'auto'}: C3 torus 216")
+        dg = c.gen_grid3d(216)
+        try:
+            parent, depth, st = dg.bfs(0)
+            assert st["levels"] == 325 and st["launches"] == 1
+            wp, wd = oracle.torus_closed_form(216)
+            assert np.array_equal(depth, wd) and np.array_equal(parent, wp)
+        finally:
+            dg.free()
+        say(f"persistent, grid {grid or 'auto'}: C4 rmat24")
+        dg = c.gen_rmat(24, 2 ** 28, 0.57, 0.19, 0.19, seed=1, wide_offsets=True)
+        try:
+            src, _ = nonisolated_sources(dg, 2, 2)
+            for s in src:
+                p1, d1, st = dg.bfs(int(s))
+                assert st["launches"] == 1
+                r = dg.check(int(s), p1, d1)
+                assert r["valid"] == 1 and r["max_rule_bad"] == 0 and r["depth_bad"] == 0, r
+                c.set_tuning(persist=0)
+                p0, d0, _ = dg.bfs(int(s))
+                c.set_tuning(persist=1)
+                assert np.array_equal(p1, p0) and np.array_equal(d1, d0)
+        finally:
+            dg.free()
