@@ -172,6 +172,24 @@ typedef struct {
 int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
              int32_t* depth, uint32_t out_flags, pbbs_bfs_stats* stats);
 
+/* Allocation-free variant of pbbs_bfs.  No reference counterpart: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The BFS's working
+ * memory (bitmaps, frontier queues, control block) is a static 32 MiB device
+ * array of the library, one per GPU, held until the process ends; the call
+ * makes no hipMalloc, hipFree or other allocating call.  Limits and rules:
+ *   - n <= 2^25 vertices (PBBS_ERR_LIMIT beyond);
+ *   - parent and depth are DEVICE buffers of n entries, both required;
+ *   - the graph must have been prepared on this context
+ *     (pbbs_graph_prepare with PBBS_PREPARE_SINGLE), else PBBS_ERR_STATE;
+ *   - every call on one GPU shares that GPU's array: calls that overlap on one
+ *     GPU (other threads or contexts) race, so the caller serialises them.
+ * Outputs are bit-identical to pbbs_bfs; the call returns once they are in
+ * HBM.  Frontier queues are smaller than pbbs_bfs's for large n (the rest of
+ * a large frontier is expanded from its bitmap), so timings may differ. */
+int pbbs_bfs_noalloc(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
+                     int32_t* depth, pbbs_bfs_stats* stats);
+
 /* ---- batched BFS (up to 64 sources in one traversal) ------------------- */
 #define PBBS_MAX_BATCH 64
 

@@ -154,6 +154,7 @@ SIGNATURES = [
     ("pbbs_bfs_multi", C.c_int, [_P, C.c_int32, _P, _P, _P, _P, C.c_uint32,
                                  C.POINTER(BfsStats)]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
+    ("pbbs_bfs_noalloc", C.c_int, [_P, _P, C.c_int32, _P, _P, C.POINTER(BfsStats)]),
     ("pbbs_bfs_batch", C.c_int,
      [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_profile", C.c_int,

@@ -335,6 +335,10 @@ int pbbs_ctx_create(int device, pbbs_ctx** out) {
     pbbs_ctx_destroy(c);
     return fail(PBBS_ERR_HIP, "context setup on device %d failed", device);
   }
+  if (scratch_init(device) != PBBS_OK) {
+    pbbs_ctx_destroy(c);
+    return PBBS_ERR_HIP;   // scratch_init's message stands
+  }
   c->num_cus = prop.multiProcessorCount;
   if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess ||
       c->wall_khz <= 0)
@@ -1877,6 +1881,62 @@ extern "C" int pbbs_bfs_multi(pbbs_ctx* c, int32_t njobs, pbbs_graph* const* gra
   }
   return PBBS_OK;
 }
+
+// ----------------------------------------------- BFS in a caller's memory block
+namespace pbbs {
+
+// A single-source BFS whose working memory is the device block [blk, blk +
+// bytes), laid out as a multi-graph job's workspace without outputs; the
+// frontier queues get what the block leaves (at most queue_cap(n,
+// queue_div) entries per sub-queue; a frontier that does not fit is expanded
+// from its bitmap, so the outputs do not change).  Nothing is allocated: the
+// graph's search data must exist (pbbs_graph_prepare), the levels run in one
+// persistent launch or as plain launches (no captured graphs), and the
+// outputs go to the caller's device buffers.  The init kernel clears every
+// part of the block the levels read before writing it.
+int bfs_in_block(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
+                 unsigned char* blk, int64_t bytes, int64_t max_n, pbbs_bfs_stats* stats) {
+  clear_error();
+  if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
+  if (!d_parent || !d_depth)
+    return fail(PBBS_ERR_INVALID, "parent and depth must be device buffers of n entries");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (g->n > max_n)
+    return fail(PBBS_ERR_LIMIT, "n=%lld exceeds the %lld vertices the static scratch serves",
+                (long long)g->n, (long long)max_n);
+  if (s < 0 || (int64_t)s >= g->n)
+    return fail(PBBS_ERR_INVALID, "source %d outside [0, %lld)", s, (long long)g->n);
+  if (!blk) return fail(PBBS_ERR_STATE, "no scratch block on device %d", c->device);
+  const int64_t fixed = job_layout_q(g->n, 0, false).total;
+  const int64_t per_entry = 2 * (int64_t)kSubQueues * (sizeof(int32_t) + sizeof(uint32_t));
+  const int64_t room = (bytes - fixed - 2 * kJobAlign) / per_entry;
+  const int64_t qcap = std::min(queue_cap(g->n, c->tuning.queue_div), room);
+  if (qcap < kWave)
+    return fail(PBBS_ERR_LIMIT, "n=%lld leaves no room for the frontier queues in %lld bytes",
+                (long long)g->n, (long long)bytes);
+  const JobLayout L = job_layout_q(g->n, qcap, false);
+  if (L.total > bytes) return fail(PBBS_ERR_LIMIT, "workspace layout exceeds the scratch block");
+  SingleWs ws;
+  ws.bits = reinterpret_cast<unsigned long long*>(blk + L.bits);
+  ws.q = reinterpret_cast<int32_t*>(blk + L.q);
+  ws.qe = reinterpret_cast<uint32_t*>(blk + L.qe);
+  ws.qcap = qcap;
+  ws.st = reinterpret_cast<BfsState*>(blk + L.st);
+  ws.wg = reinterpret_cast<uint32_t*>(blk + L.wg);
+  ws.args = blk + L.args;
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = g->off64 ? begin_bfs<int64_t>(c, g, s, d_parent, d_depth, ws, false)
+                    : begin_bfs<uint32_t>(c, g, s, d_parent, d_depth, ws, false);
+  bool done = false;
+  while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
+  if (rc != PBBS_OK) {
+    release_persist(c);
+    return rc;
+  }
+  return stats ? single_stats(c, stats) : PBBS_OK;
+}
+
+}  // namespace pbbs
 
 // ------------------------------------------------------------ graph download
 extern "C" int pbbs_graph_download(const pbbs_graph* g, pbbs_csr* out) {
