@@ -90,9 +90,11 @@ typedef struct {
                                the rest in the frontier bitmap only, and the level after it
                                expands that bitmap (same outputs, bounded workspace) */
     int32_t persist;        /* 1 = a single-source BFS runs every level in one persistent
-                               launch whose workgroups meet at a grid barrier between levels;
-                               one such launch per device at a time, others run one level per
-                               launch.  0 = one level per launch (default) */
+                               launch whose workgroups (grid_blocks, else one per CU) meet at
+                               a grid barrier between levels; one such launch per device at a
+                               time, others run one level per launch.  0 = one level per
+                               launch.  2 (default) = persistent once the graph's previous run
+                               needed 64 levels or more (deep graphs), else one per launch */
     int32_t persist_wait_us;/* longest wait at a barrier of a persistent launch (default
                                200000); a launch that waits longer (its grid could not gather,
                                e.g. on a device shared with other processes) gives up and the

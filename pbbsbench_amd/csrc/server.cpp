@@ -370,6 +370,14 @@ int pbbs_server_create(int device, const pbbs_server_config* cfg, pbbs_server** 
   s->slots.resize(c.slots);
   for (Slot& sl : s->slots) {
     rc = pbbs_ctx_create(device, &sl.ctx);
+    if (rc == PBBS_OK) {
+      // slots' runs overlap on the device, so a persistent launch could not
+      // count on its grid being resident at once: one level per launch
+      pbbs_tuning t;
+      rc = pbbs_tuning_get(sl.ctx, &t);
+      t.persist = 0;
+      if (rc == PBBS_OK) rc = pbbs_tuning_set(sl.ctx, &t);
+    }
     if (rc == PBBS_OK && hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming) != hipSuccess)
       rc = fail(PBBS_ERR_HIP, "event creation failed");
     if (rc != PBBS_OK) {
