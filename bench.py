@@ -564,6 +564,57 @@ def run_serve(args, nested):
     return res, exact
 
 
+# ---------------------------------------------------------------- multi-graph
+def run_multi(args):
+    """Several graphs of uneven size in one launch: the eight serving graphs
+    (RMAT scales 16..22, 16 draws per vertex, a=.57 b=.19 c=.19), one seeded
+    non-isolated source each, as one pbbs_bfs_multi launch (each job on its
+    own share of one persistent grid) against the eight pbbs_bfs calls one
+    after another.  Outputs in HBM while timed; parity checked after."""
+    import pbbsbench_amd as pb
+    rng = np.random.default_rng(args.seed + 7)
+    with pb.Context(0) as ctx:
+        dgs = [ctx.gen_rmat(sc, 16 * 2 ** sc, 0.57, 0.19, 0.19, seed=sd) for sc, sd in SERVE_GRAPHS]
+        hosts = [d.download() for d in dgs]
+        srcs = [int(rng.choice(np.flatnonzero(h.degrees() > 0))) for h in hosts]
+        for d in dgs:
+            d.prepare(single=True, batch=False)
+        for _ in range(max(args.warmup, 1)):
+            ctx.bfs_multi_device(dgs, srcs)
+            for d, s in zip(dgs, srcs):
+                d.bfs_device(s)
+        steps = max(args.steps, 1)
+        t0 = time.perf_counter()
+        mdev = [ctx.bfs_multi_device(dgs, srcs)[0]["device_ms"] for _ in range(steps)]
+        multi_ms = (time.perf_counter() - t0) / steps * 1e3
+        t0 = time.perf_counter()
+        sdev = []
+        for _ in range(steps):
+            sdev.append(sum(d.bfs_device(s)["device_ms"] for d, s in zip(dgs, srcs)))
+        seq_ms = (time.perf_counter() - t0) / steps * 1e3
+        out = ctx.bfs_multi(dgs, srcs)
+        exact = True
+        edges = 0
+        for (p, dp, st), d, s in zip(out, dgs, srcs):
+            wp, wd, ws = d.bfs(s)
+            exact &= bool(np.array_equal(p, wp)) and bool(np.array_equal(dp, wd))
+            edges += ws["edges_reached"]
+        for d in dgs:
+            d.free()
+    return {
+        "workload": "serve8 graphs, one query each",
+        "graphs": [f"rmat{sc} seed {sd}" for sc, sd in SERVE_GRAPHS],
+        "sources": srcs,
+        "ms_per_launch": round(multi_ms, 4),
+        "event_ms_per_launch": round(float(np.mean(mdev)), 4),
+        "sequential_wall_ms_8": round(seq_ms, 4),
+        "sequential_event_ms_8": round(float(np.mean(sdev)), 4),
+        "multi_vs_sequential": round(seq_ms / multi_ms, 3),
+        "gteps": round(edges / (multi_ms / 1e3) / 1e9, 2),
+        "parity": "bit-exact vs one pbbs_bfs per graph" if exact else "MISMATCH",
+    }, exact
+
+
 # ----------------------------------------------------------------------- ingest
 def bench_ingest(args):
     """Graph file -> HBM.  The workload's graph is built on the GPU, downloaded
@@ -666,13 +717,16 @@ def main():
                     help="budget of the CPU-baseline timed rounds after its 1 s warm-up (0 = skip)")
     ap.add_argument("--td-to-bu-div", type=int, default=None)
     ap.add_argument("--bu-to-td-div", type=int, default=None)
-    ap.add_argument("--mode", default="single", choices=["single", "batch", "serve", "ingest"],
+    ap.add_argument("--mode", default="single",
+                    choices=["single", "batch", "serve", "multi", "ingest"],
                     help="single: one-source BFS with the batched and serving parts nested in "
                          "its line (default); batch: 64 sources in one traversal vs 64 "
                          "sequential runs on RMAT 2^22 alone; serve: Poisson query stream over "
                          "8 resident RMAT graphs alone; ingest: graph file (text / binary) into HBM")
     ap.add_argument("--no-batched", action="store_true", help="single: skip the batched part")
     ap.add_argument("--no-serving", action="store_true", help="single: skip the serving part")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="single: skip the several-graphs-in-one-launch part")
     ap.add_argument("--no-tuning-table", action="store_true",
                     help="run on the library defaults instead of the committed tuning table")
     ap.add_argument("--qps", type=float, default=0.0,
@@ -712,6 +766,14 @@ def main():
             "roofline": res["roofline"], "parity": res["parity"]},
             **({"cpu_baseline": res.pop("cpu_baseline")} if "cpu_baseline" in res else {}))))
         sys.exit(0 if exact else 1)
+    if args.mode == "multi":
+        res, exact = run_multi(args)
+        print(json.dumps(line({
+            "metric": "several graphs of uneven size in one launch vs one launch sequence each",
+            "value": res["multi_vs_sequential"], "unit": "x", "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_launch"],
+            "higher_is_better": True, "config": res, "parity": res["parity"]})))
+        sys.exit(0 if exact else 1)
     if args.mode == "serve":
         res, exact = run_serve(args, nested=False)
         print(json.dumps(line({
@@ -727,6 +789,10 @@ def main():
         exact &= ok
     if not args.no_serving:
         result["serving"], ok = run_serve(args, nested=True)
+        exact &= ok
+    if not args.no_multi:
+        say("multi-graph launch")
+        result["multi_graph"], ok = run_multi(args)
         exact &= ok
     print(json.dumps(result))
     if not exact:

@@ -7,7 +7,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpbbs_bfs.so")
+# PBBS_BFS_LIB names another build of the library (e.g. one compiled with
+# different launch-shape constants, for timing it against this one).
+LIB_PATH = os.environ.get("PBBS_BFS_LIB") or os.path.join(_HERE, "libpbbs_bfs.so")
 
 PBBS_OK = 0
 PBBS_ERR_INVALID = -1

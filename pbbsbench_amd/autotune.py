@@ -10,8 +10,10 @@ measured per graph class and kept in a small JSON tuning table:
                                 "ms": 0.53, "default_ms": 0.54, "graph": {...}}, ...}}
 
 A graph's class is its degree shape ("skewed" when the largest degree is more
-than 16 times the mean degree of the non-isolated vertices, else "uniform")
-and its scale (log2 n, rounded).  The search is a coordinate descent over a
+than 16 times the mean degree of the non-isolated vertices, else "uniform"),
+its scale (log2 n, rounded) and its mean degree (log2 of the mean degree of
+the non-isolated vertices, rounded): "uniform-s23-d3" is the 216^3 torus,
+"uniform-s23-d4" a random-local graph of 10M vertices and degree ~20.  The search is a coordinate descent over a
 small grid per parameter, starting from the library defaults, so the default
 setting is always among the measured candidates.  Outputs are bit-identical
 under every setting (parents and depths depend only on the graph and the
@@ -42,6 +44,7 @@ SINGLE_GRID = {
     "bu_to_td_div": [5, 10, 20, 40, 80, 160],
     "chunk_levels": [4, 16, 64],
     "bu_serial_depth": [32, 64, 128, 256],
+    "persist": [0, 1, 2],
 }
 BATCH_GRID = {
     "batch_pull_div": [3, 4, 6, 10, 14, 20],
@@ -55,7 +58,8 @@ def graph_class(info):
     nz = max(n - int(info["isolated"]), 1)
     mean = max(int(info["m"]) / nz, 1.0)
     shape = "skewed" if int(info["max_degree"]) > 16 * mean else "uniform"
-    return f"{shape}-s{int(round(math.log2(max(n, 2))))}"
+    return (f"{shape}-s{int(round(math.log2(max(n, 2))))}"
+            f"-d{int(round(math.log2(mean)))}")
 
 
 def coordinate_descent(grid, defaults, measure, rounds=2, min_gain=0.01):
@@ -75,6 +79,7 @@ def coordinate_descent(grid, defaults, measure, rounds=2, min_gain=0.01):
         key = tuple(sorted(p.items()))
         if key not in seen:
             ms, digest = measure(dict(p))
+            print(f"[autotune] {dict(p)}: {ms:.4f} ms", file=sys.stderr, flush=True)
             if "digest" not in ref:
                 ref["digest"] = digest
             elif digest != ref["digest"]:
@@ -239,6 +244,8 @@ def main(argv=None):
     g.add_argument("--rmat", type=int, metavar="SCALE",
                    help="RMAT graph of 2^SCALE vertices, 16 * 2^SCALE draws, a,b,c = .57,.19,.19")
     g.add_argument("--grid", type=int, metavar="D", help="D x D x D torus")
+    g.add_argument("--randlocal", type=int, metavar="N",
+                   help="random-local graph of N vertices, 10 draws each, seed 1")
     g.add_argument("--graph", metavar="FILE", help="AdjacencyGraph text or PBBSCSR1 binary file")
     ap.add_argument("--out", required=True, help="tuning table (JSON); updated in place")
     ap.add_argument("--wide-offsets", action="store_true", help="keep int64 offsets in HBM")
@@ -255,6 +262,9 @@ def main(argv=None):
                               wide_offsets=args.wide_offsets)
         elif args.grid is not None:
             dg = ctx.gen_grid3d(args.grid, wide_offsets=args.wide_offsets)
+        elif args.randlocal is not None:
+            from .core import randlocal
+            dg = ctx.upload(randlocal(args.randlocal, 10, seed=1), wide_offsets=args.wide_offsets)
         else:
             dg = ctx.read_graph(args.graph, wide_offsets=args.wide_offsets)
         key, entry = tune(ctx, dg, args.runs, args.sources, args.batch, args.seed)

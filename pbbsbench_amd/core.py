@@ -261,6 +261,20 @@ class Context:
         check(lib().pbbs_bfs_multi(self._h, J, gh, _ptr(src), pp, dp, 0, st), "BFS multi")
         return [(par[j], dep[j] if depth else None, st[j].as_dict()) for j in range(J)]
 
+    def bfs_multi_device(self, graphs, sources):
+        """bfs_multi with the outputs left in HBM (the context's multi-graph
+        workspace); returns the per-job stats dicts (device_ms is the whole
+        launch's)."""
+        J = len(graphs)
+        if J != len(sources):
+            raise ValueError("one source per graph")
+        gh = (C.c_void_p * J)(*[g._h for g in graphs])
+        src = np.ascontiguousarray(sources, dtype=np.int32)
+        st = (_lib.BfsStats * J)()
+        check(lib().pbbs_bfs_multi(self._h, J, gh, _ptr(src), None, None, _lib.PBBS_OUT_DEVICE,
+                                   st), "BFS multi")
+        return [st[j].as_dict() for j in range(J)]
+
     def gen_rmat(self, scale, draws, a, b, c, seed=1, permute_seed=0, wide_offsets=False):
         """rmat(...) built on the GPU, straight into HBM; the same graph bit for
         bit as ctx.upload(rmat(...)) (testData/graphData/rMatGraph.C:33-79)."""

@@ -164,6 +164,9 @@ struct MbfsArgs {
   int32_t td_to_bu_div;        // 0 = push levels only
   int32_t pull_div;            // pull when (|F| + edges(F)) * pull_div > m
   int32_t num_cus;             // CUs of the device (workgroup slot = blockIdx / num_cus)
+  int32_t preset_parents;      // 1: parent already holds -1 everywhere (filled ahead of
+                               // the batch), so the clear kernel leaves it alone
+  int32_t pad;
 };
 
 }  // namespace pbbs

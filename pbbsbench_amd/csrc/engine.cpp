@@ -74,6 +74,7 @@ hipError_t launch_share_bounds(const unsigned long long* prefix, long long nword
 template <typename OffT>
 hipError_t launch_row_tops(const OffT* off, const int32_t* adj, long long n, uint32_t* out,
                            int grid, hipStream_t stream);
+hipError_t launch_fill_minus1(int32_t* p, long long count, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_graph_symmetry(const OffT* off, const int32_t* adj, long long n, long long m,
                                  unsigned long long* counters, int grid, hipStream_t stream);
@@ -175,8 +176,16 @@ struct pbbs_ctx {
   void* b_args = nullptr;                // device copy of MbfsArgs<OffT>
   int32_t* b_sources = nullptr;          // kMaxSources
   int64_t cap_bout = 0;                  // int32 entries of each output buffer
-  int32_t* b_parent = nullptr;
+  int32_t* b_parent = nullptr;          // parents of the last batched run
   int32_t* b_depth = nullptr;
+  // A second parent buffer, filled with -1 on the side stream while a batch
+  // runs so that the next batch needs no clear of its parents (the two swap
+  // each batch); alt_cleared = entries of it known to hold -1.
+  int32_t* b_parent_alt = nullptr;
+  int64_t alt_cleared = 0;
+  bool prefill = true;                   // false in server slots (their runs overlap)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_alt = nullptr;
   uint8_t* b_dlev = nullptr;             // [cap_bn][64] per-vertex level table
   int32_t* b_q = nullptr;                // union queues: q0 | q1, kSubQueues regions of cap_bn
   uint32_t* b_qe = nullptr;
@@ -331,7 +340,9 @@ int pbbs_ctx_create(int device, pbbs_ctx** out) {
       hipHostMalloc((void**)&c->host_status, 64, hipHostMallocCoherent) != hipSuccess ||
       hipHostMalloc((void**)&c->host_state, sizeof(BfsState), hipHostMallocDefault) != hipSuccess ||
       hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess ||
-      hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess) {
+      hipEventCreate(&c->ev_c0) != hipSuccess || hipEventCreate(&c->ev_c1) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_alt, hipEventDisableTiming) != hipSuccess) {
     pbbs_ctx_destroy(c);
     return fail(PBBS_ERR_HIP, "context setup on device %d failed", device);
   }
@@ -352,6 +363,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   if (!c) return PBBS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   release_persist(c);
   {
     std::lock_guard<std::mutex> lk(c->graphs_mu);
@@ -370,6 +382,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   (void)hipFree(c->b_args);
   (void)hipFree(c->b_sources);
   (void)hipFree(c->b_parent);
+  (void)hipFree(c->b_parent_alt);
   (void)hipFree(c->b_depth);
   (void)hipFree(c->w_state);
   (void)hipFree(c->w_wg_level);
@@ -385,7 +398,9 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   if (c->ev_end) (void)hipEventDestroy(c->ev_end);
   if (c->ev_c0) (void)hipEventDestroy(c->ev_c0);
   if (c->ev_c1) (void)hipEventDestroy(c->ev_c1);
+  if (c->ev_alt) (void)hipEventDestroy(c->ev_alt);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->side) (void)hipStreamDestroy(c->side);
   delete c;
   return PBBS_OK;
 }
@@ -1301,14 +1316,24 @@ static int ensure_batch_workspace(pbbs_ctx* c, int64_t n, int64_t out_entries, b
     c->cap_bn = n;
   }
   if (need_out && out_entries > c->cap_bout) {
+    HIP_TRY(hipStreamSynchronize(c->side));   // a fill of the old alt buffer may be running
     (void)hipFree(c->b_parent);
+    (void)hipFree(c->b_parent_alt);
     (void)hipFree(c->b_depth);
-    c->b_parent = c->b_depth = nullptr;
+    c->b_parent = c->b_parent_alt = c->b_depth = nullptr;
+    c->alt_cleared = 0;
     c->cap_bout = 0;
     if (hipMalloc(&c->b_parent, out_entries * sizeof(int32_t)) != hipSuccess ||
-        hipMalloc(&c->b_depth, out_entries * sizeof(int32_t)) != hipSuccess)
+        hipMalloc(&c->b_parent_alt, out_entries * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc(&c->b_depth, out_entries * sizeof(int32_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(c->b_parent);
+      (void)hipFree(c->b_parent_alt);
+      (void)hipFree(c->b_depth);
+      c->b_parent = c->b_parent_alt = c->b_depth = nullptr;
       return fail(PBBS_ERR_NOMEM, "batched output buffers (%lld entries) failed",
                   (long long)out_entries);
+    }
     c->cap_bout = out_entries;
   }
   return PBBS_OK;
@@ -1353,10 +1378,24 @@ static int get_batch_graph(pbbs_ctx* c, bool sorted, int grid, int K, hipGraphEx
   return PBBS_OK;
 }
 
+// own_out: the outputs go to the context's own buffers (d_parent and d_depth
+// are ignored), which lets the parents come pre-filled with -1 (see
+// b_parent_alt); otherwise the clear kernel fills the caller's parents.
 template <typename OffT>
 static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent,
-                       int32_t* d_depth) {
+                       int32_t* d_depth, bool own_out) {
   const int64_t n = g->n;
+  const int64_t entries = (int64_t)nsrc * n;
+  bool preset = false;
+  if (own_out) {
+    if (c->b_parent_alt && c->alt_cleared >= entries) {
+      std::swap(c->b_parent, c->b_parent_alt);
+      c->alt_cleared = 0;   // it now holds the previous batch's parents
+      preset = true;
+    }
+    d_parent = c->b_parent;
+    d_depth = c->b_depth;
+  }
   MbfsArgs<OffT> a;
   std::memset(&a, 0, sizeof a);
   a.off = static_cast<const OffT*>(g->d_off);
@@ -1384,6 +1423,7 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   a.td_to_bu_div = c->tuning.td_to_bu_div;
   a.pull_div = c->tuning.batch_pull_div;
   a.num_cus = c->num_cus;
+  a.preset_parents = preset ? 1 : 0;
   pbbs_ctx::Run& r = c->run;
   r.kind = 1;
   r.off64 = g->off64;
@@ -1417,11 +1457,21 @@ static int begin_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_pare
   hs[0] = 0;
   hs[1] = 0;
   HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  if (preset) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_alt, 0));   // inside the timed span
   HIP_TRY(launch_mbfs_setup<OffT>(a, static_cast<MbfsArgs<OffT>*>(c->b_args), setup_grid,
                                   c->stream));
   c->b_last_args_off64 = g->off64;
   std::memcpy(c->b_last_args, &a, sizeof a);
-  return launch_chunk(c);
+  const int rc = launch_chunk(c);
+  if (rc != PBBS_OK) return rc;
+  if (own_out && c->b_parent_alt && c->prefill) {
+    // the other buffer's contents (the previous batch's outputs) are no
+    // longer needed: fill it for the next batch while this one runs
+    HIP_TRY(launch_fill_minus1(c->b_parent_alt, entries, std::max(1, c->num_cus / 8), c->side));
+    HIP_TRY(hipEventRecord(c->ev_alt, c->side));
+    c->alt_cleared = entries;
+  }
+  return PBBS_OK;
 }
 
 static int batch_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
@@ -1442,8 +1492,8 @@ static int batch_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
 
 template <typename OffT>
 static int run_batch(pbbs_ctx* c, pbbs_graph* g, int32_t nsrc, int32_t* d_parent, int32_t* d_depth,
-                     pbbs_bfs_stats* stats) {
-  int rc = begin_batch<OffT>(c, g, nsrc, d_parent, d_depth);
+                     bool own_out, pbbs_bfs_stats* stats) {
+  int rc = begin_batch<OffT>(c, g, nsrc, d_parent, d_depth, own_out);
   bool done = false;
   while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
   if (rc != PBBS_OK) return rc;
@@ -1471,13 +1521,15 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
   const int64_t entries = (int64_t)nsrc * g->n;
   int rc = ensure_batch_workspace(c, g->n, entries, !(dev_out && parents && depths));
   if (rc != PBBS_OK) return rc;
-  int32_t* dp = (dev_out && parents) ? parents : c->b_parent;
-  int32_t* dd = (dev_out && depths) ? depths : c->b_depth;
+  // outputs to the caller's device buffers only when both are given
+  const bool own_out = !(dev_out && parents && depths);
   HIP_TRY(hipMemcpyAsync(c->b_sources, sources, nsrc * sizeof(int32_t), hipMemcpyHostToDevice,
                          c->stream));
-  rc = g->off64 ? run_batch<int64_t>(c, g, nsrc, dp, dd, stats)
-                : run_batch<uint32_t>(c, g, nsrc, dp, dd, stats);
+  rc = g->off64 ? run_batch<int64_t>(c, g, nsrc, parents, depths, own_out, stats)
+                : run_batch<uint32_t>(c, g, nsrc, parents, depths, own_out, stats);
   if (rc != PBBS_OK) return rc;
+  int32_t* dp = own_out ? c->b_parent : parents;
+  int32_t* dd = own_out ? c->b_depth : depths;
   if (!dev_out && (parents || depths)) {
     HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
     if (parents)
@@ -1510,7 +1562,7 @@ int64_t bfs_workspace_bytes(int64_t n, int32_t nsrc, int32_t queue_div) {
   if (nsrc <= 1)
     return 8 * n + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * queue_cap(n, queue_div) * 8;
   return 5 * n * 8 + 4 * nwords * 8 + n * kMaxSources + 2 * (int64_t)kSubQueues * n * 8 +
-         2 * (int64_t)nsrc * n * 4;
+         3 * (int64_t)nsrc * n * 4;
 }
 
 // Device bytes the context's workspace buffers hold now (they grow with the
@@ -1524,6 +1576,7 @@ int64_t ctx_workspace_bytes(const pbbs_ctx* c) {
     b += 5 * c->cap_bn * 8 + 4 * bwords * 8 + c->cap_bn * kMaxSources +
          2 * (int64_t)kSubQueues * c->cap_bn * 8;
   if (c->b_parent) b += 2 * c->cap_bout * 4;
+  if (c->b_parent_alt) b += c->cap_bout * 4;
   return b;
 }
 
@@ -1543,7 +1596,7 @@ int64_t ctx_workspace_bytes_after(const pbbs_ctx* c, int64_t n, int32_t nsrc) {
   int64_t b = 0;
   if (cn) b += 8 * cn + 4 * nwords * 8 + 2 * (int64_t)kSubQueues * cq * 8;
   if (bn) b += 5 * bn * 8 + 4 * bwords * 8 + bn * kMaxSources + 2 * (int64_t)kSubQueues * bn * 8;
-  if (bo) b += 2 * bo * 4;
+  if (bo) b += 3 * bo * 4;   // parents, the pre-filled second parent buffer, depths
   return b;
 }
 
@@ -1551,13 +1604,17 @@ int64_t ctx_workspace_bytes_after(const pbbs_ctx* c, int64_t n, int32_t nsrc) {
 void ctx_trim(pbbs_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  (void)hipStreamSynchronize(c->side);
   free_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_q);
   (void)hipFree(c->b_qe);
   (void)hipFree(c->b_parent);
+  (void)hipFree(c->b_parent_alt);
   (void)hipFree(c->b_depth);
+  c->b_parent_alt = nullptr;
+  c->alt_cleared = 0;
   c->b_masks = nullptr;
   c->b_dlev = nullptr;
   c->b_q = nullptr;
@@ -1568,6 +1625,7 @@ void ctx_trim(pbbs_ctx* c) {
 }
 
 int32_t ctx_queue_div(const pbbs_ctx* c) { return c->tuning.queue_div; }
+void ctx_no_prefill(pbbs_ctx* c) { c->prefill = false; }
 
 hipStream_t ctx_stream(pbbs_ctx* c) { return c->stream; }
 int ctx_device(const pbbs_ctx* c) { return c->device; }
@@ -1605,8 +1663,8 @@ int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc
   if (rc != PBBS_OK) return rc;
   HIP_TRY(hipMemcpyAsync(c->b_sources, sources, nsrc * sizeof(int32_t), hipMemcpyHostToDevice,
                          c->stream));
-  return g->off64 ? begin_batch<int64_t>(c, g, nsrc, c->b_parent, c->b_depth)
-                  : begin_batch<uint32_t>(c, g, nsrc, c->b_parent, c->b_depth);
+  return g->off64 ? begin_batch<int64_t>(c, g, nsrc, nullptr, nullptr, true)
+                  : begin_batch<uint32_t>(c, g, nsrc, nullptr, nullptr, true);
 }
 
 int async_poll(pbbs_ctx* c, bool* done) { return poll_run(c, false, done); }

@@ -37,6 +37,7 @@ int64_t ctx_workspace_bytes(const pbbs_ctx* c);
 int64_t ctx_workspace_bytes_after(const pbbs_ctx* c, int64_t n, int32_t nsrc);
 void ctx_trim(pbbs_ctx* c);
 int32_t ctx_queue_div(const pbbs_ctx* c);
+void ctx_no_prefill(pbbs_ctx* c);
 int async_begin(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources, int32_t nsrc);
 int async_poll(pbbs_ctx* c, bool* done);
 int async_outputs(pbbs_ctx* c, const int32_t** parent, const int32_t** depth);
@@ -377,6 +378,8 @@ int pbbs_server_create(int device, const pbbs_server_config* cfg, pbbs_server** 
       rc = pbbs_tuning_get(sl.ctx, &t);
       t.persist = 0;
       if (rc == PBBS_OK) rc = pbbs_tuning_set(sl.ctx, &t);
+      // nor does a batched run fill the next run's parents beside itself
+      pbbs::ctx_no_prefill(sl.ctx);
     }
     if (rc == PBBS_OK && hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming) != hipSuccess)
       rc = fail(PBBS_ERR_HIP, "event creation failed");

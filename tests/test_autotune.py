@@ -16,14 +16,20 @@ def host_info(g):
 
 def test_graph_class():
     assert autotune.graph_class(host_info(pb.rmat(12, 16 * 4096, .57, .19, .19, seed=1))) == \
-        "skewed-s12"
-    assert autotune.graph_class(host_info(pb.grid3d(16))) == "uniform-s12"
-    assert autotune.graph_class(host_info(pb.randlocal(100000, 10, seed=1))) == "uniform-s17"
+        "skewed-s12-d5"
+    assert autotune.graph_class(host_info(pb.grid3d(16))) == "uniform-s12-d3"
+    assert autotune.graph_class(host_info(pb.randlocal(100000, 10, seed=1))) == "uniform-s17-d4"
     # rounding of the scale and the 16x-mean cut
     assert autotune.graph_class({"n": 3000, "m": 30000, "isolated": 0, "max_degree": 160}) == \
-        "uniform-s12"
+        "uniform-s12-d3"
     assert autotune.graph_class({"n": 3000, "m": 30000, "isolated": 0, "max_degree": 161}) == \
-        "skewed-s12"
+        "skewed-s12-d3"
+    # the 216^3 torus and a random-local graph of 10M vertices share a scale
+    # but not a class (the mean degree tells them apart)
+    assert autotune.graph_class({"n": 10077696, "m": 60466176, "isolated": 0,
+                                 "max_degree": 6}) == "uniform-s23-d3"
+    assert autotune.graph_class({"n": 10_000_000, "m": 196714352, "isolated": 0,
+                                 "max_degree": 40}) == "uniform-s23-d4"
 
 
 def separable_cost(opt):

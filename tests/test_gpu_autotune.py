@@ -28,7 +28,7 @@ def test_tune_apply_and_parity(ctx, tmp_path, kind):
         key, entry = autotune.tune(ctx, dg, runs=2, sources=2, batch=True, seed=5,
                                    single_grid=SMALL_SINGLE, batch_grid=SMALL_BATCH)
         assert key == autotune.graph_class(dg.info)
-        assert key.startswith("skewed-s14" if kind == "rmat14" else "uniform-s14")
+        assert key.startswith("skewed-s14-" if kind == "rmat14" else "uniform-s14-")
         p = entry["params"]
         for k, vals in list(SMALL_SINGLE.items()) + list(SMALL_BATCH.items()):
             assert p[k] in set(vals) | {saved[k]}

@@ -141,3 +141,38 @@ def test_batch_deep_levels(pull):
     with pb.Context(0) as c:
         c.set_tuning(batch_pull_div=pull)
         check_batch(c, g, [0, 350, 699, 5, 0])
+
+
+def test_batch_prefilled_parent_buffers():
+    """Batches in a row on one context: each batch's parents come from the
+    buffer filled with -1 on the side stream during the batch before, when
+    it is large enough, else the clear kernel fills it.  Sizes go up and
+    down, graphs change, and a batch into the caller's device buffers sits in
+    between; every row equals the oracle."""
+    g1 = pb.rmat(12, 16 * 4096, 0.57, 0.19, 0.19, seed=21)
+    g2 = pb.grid3d(11)
+    nz1 = np.flatnonzero(g1.degrees() > 0)
+    with pb.Context(0) as c:
+        d1, d2 = c.upload(g1), c.upload(g2)
+        try:
+            plan = [(d1, g1, nz1[:64]), (d1, g1, nz1[64:128]), (d1, g1, nz1[:8]),
+                    (d2, g2, np.arange(0, 640, 10)), (d1, g1, nz1[200:264]),
+                    ("dev", None, None), (d1, g1, nz1[300:364]), (d1, g1, nz1[:3])]
+            for step, (dg, g, src) in enumerate(plan):
+                if dg == "dev":
+                    # caller's device buffers: the ctx buffers of a single-source run
+                    # hold n entries, so use a one-source batch into them
+                    s = int(nz1[5])
+                    d1.bfs_device(s)
+                    pp, dp = c.outputs(batched=False)
+                    d1.bfs_batch_device(np.array([s], dtype=np.int32), pp, dp)
+                    continue
+                say(f"step {step}: n={g.n} k={len(src)}")
+                P, D, _ = dg.bfs_batch(src)
+                for i, s in enumerate(src):
+                    wp, wd = oracle.serial_bfs(g.offsets, g.adj, int(s))
+                    assert np.array_equal(P[i], wp), f"step {step} row {i}: parent"
+                    assert np.array_equal(D[i], wd), f"step {step} row {i}: depth"
+        finally:
+            d1.free()
+            d2.free()
