@@ -27,7 +27,10 @@
 #include <stdint.h>
 #include <climits>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "bfs_device.hpp"
 
@@ -51,21 +54,6 @@ __device__ __forceinline__ T* gen(gptr<T> p) {
   return (T*)p;
 }
 
-#ifdef PBBS_WAVE_TIMING
-// Diagnostic build only: per-wavefront wall-clock stamps (start, end of the
-// first pass, end) of the first 16 levels, for 16384 wavefronts.
-constexpr int kTimingLevels = 16;
-constexpr int kTimingWaves = 16384;
-__device__ unsigned long long g_wave_times[kTimingLevels * kTimingWaves * 3];
-__device__ __forceinline__ void stamp(int k, int slot) {
-  const long long gw = (long long)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (k < kTimingLevels && gw < kTimingWaves && __lane_id() == 0)
-    g_wave_times[((long long)k * kTimingWaves + gw) * 3 + slot] = wall_clock64();
-}
-#define PBBS_STAMP(k, slot) stamp((k), (slot))
-#else
-#define PBBS_STAMP(k, slot) ((void)0)
-#endif
 
 // ---------------------------------------------------------------- wave utils
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
@@ -623,10 +611,7 @@ __device__ __forceinline__ int scan_unsorted(const LevelView<OffT>& a, bool mine
 // times as many independent fetches in flight: the first bottom-up level is
 // bound by the latency of these round trips (3 measured best on RMAT 2^24,
 // against 1 and 2).
-#ifndef PBBS_BU_PER_LANE
-#define PBBS_BU_PER_LANE 3
-#endif
-constexpr int kBuPer = PBBS_BU_PER_LANE;
+constexpr int kBuPer = 3;
 constexpr int kBuCap = (kBuPer + 1) * kWave;   // queue and park capacity per wavefront
 constexpr int kTopEntries = 6;                 // neighbours held by a row-top record
 static_assert(kBuCap >= 2 * kWave, "the bottom-up and bitmap top-down lists take 64 + 63 entries");
@@ -895,7 +880,6 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
     }
   }
   if (nq) bu_pass<OffT, kSorted, kTops>(a, sb, stg, L, nq, npark);
-  PBBS_STAMP(a.k, 1);
   if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark, kTops ? kTopEntries : 4);
 }
 
@@ -1005,7 +989,6 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
     }
     return false;
   }
-  PBBS_STAMP(k, 0);
   const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
   const int mode = level_mode(*A, prev, cnt, E);
   const bool qover = st->qover[c0] != 0;
@@ -1088,18 +1071,14 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
     bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
     __builtin_amdgcn_s_setprio(0);
   }
-  PBBS_STAMP(k, 2);
   stage_flush_block(a, sb, stg, mode == kTopDown);
   return true;
 }
 
-#ifndef PBBS_LEVEL_MIN_BLOCKS
-#define PBBS_LEVEL_MIN_BLOCKS 1
-#endif
 // One level per launch: the level number comes from the workgroup's own
 // counter, so a captured sequence of launches needs no host input.
 template <typename OffT, bool kSorted, bool kTops>
-__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
+__global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
   const int k = (int)A->wg_level[blockIdx.x];
@@ -1202,7 +1181,7 @@ __device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_
 // barrier within wait_ticks gives up as described above.  On leaving, each
 // workgroup records the level it reached, as the one-level kernel does.
 template <typename OffT, bool kSorted, bool kTops>
-__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_persist_kernel(const BfsArgs<OffT>* __restrict__ A) {
+__global__ __launch_bounds__(kBlock) void bfs_persist_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
   __shared__ int stop;
@@ -1233,7 +1212,7 @@ __global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_persist_ker
 // the device at once (the host sizes it by occupancy); a job whose barrier
 // wait runs out ends as in the persistent kernel and the host runs it again.
 template <typename OffT, bool kSorted>
-__global__ __launch_bounds__(kBlock, PBBS_LEVEL_MIN_BLOCKS) void bfs_multi_kernel(
+__global__ __launch_bounds__(kBlock) void bfs_multi_kernel(
     const BfsArgs<OffT>* __restrict__ args, const uint32_t* __restrict__ wg_begin, int njobs) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
@@ -1362,10 +1341,10 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
 // per CU with 96 or 80 VGPRs) measured 4.9-5.8 ms per batch against 4.47.
 constexpr int kWinRows = 32;      // rows per window (lanes 0..31 own one each)
 constexpr int kTileStride = 65;   // LDS words per tile row: column reads hit distinct banks
-#ifndef PBBS_BATCH_UNROLL
-#define PBBS_BATCH_UNROLL 4
-#endif
-constexpr int kUnroll = PBBS_BATCH_UNROLL;   // 64-edge groups in flight per pass
+// 64-edge groups in flight per pass (3, 4, 6, 8 measured 4.49, 4.50, 4.92,
+// 5.02 ms per batch on RMAT 2^22 x 64; from 6 up occupancy falls to 3
+// wavefronts per SIMD)
+constexpr int kUnroll = 4;
 constexpr int kPass = kWave * kUnroll;
 // Work units are cut at equal sums of (degree + kRowCost) per row: a row
 // costs about as much as kRowCost edges (its window slot, the mask and
@@ -1374,18 +1353,14 @@ constexpr int kPass = kWave * kUnroll;
 constexpr int kRowCost = 32;   // swept over 8..128 on RMAT 2^22 x 64: 32 best
 // A pull level tests the frontier bitmap before each mask gather when
 // |F_k| < n / kFilterDiv.
-#ifndef PBBS_FILTER_DIV
-#define PBBS_FILTER_DIV 4
-#endif
-constexpr int kFilterDiv = PBBS_FILTER_DIV;
+// (1, 2, 4, 16 measured 4.82, 4.68, 4.48, 4.48 ms per batch)
+constexpr int kFilterDiv = 4;
 // When the previous pull left fewer than nrows / kFewRowsDiv rows with
 // sources to find, a pull level splits whole rows evenly over the wavefronts:
 // most rows are then skipped, at a cost per row, and an edge-weighted split
 // piles the many rows of low-degree id ranges on a few wavefronts.
-#ifndef PBBS_FEW_ROWS_DIV
-#define PBBS_FEW_ROWS_DIV 4
-#endif
-constexpr int kFewRowsDiv = PBBS_FEW_ROWS_DIV;
+// (2, 4, 8 measured 4.49, 4.48, 4.49 ms per batch)
+constexpr int kFewRowsDiv = 4;
 
 template <typename OffT>
 struct MbfsView {
@@ -1696,10 +1671,9 @@ __device__ __forceinline__ unsigned find_entry(gptr<const uint32_t> qe, unsigned
   return lo + (unsigned)__popcll(__ballot(pr)) - 1;
 }
 
-#ifndef PBBS_PUSH_PER_LANE
-#define PBBS_PUSH_PER_LANE 2
-#endif
-constexpr int kPushPer = PBBS_PUSH_PER_LANE;   // edges per lane per push step
+// edges per lane per push step (1, 2, 4 measured 4.49, 4.50, 4.49 ms per
+// batch: the first push is bound by its scattered atomics, not its loads)
+constexpr int kPushPer = 2;
 
 template <typename OffT>
 __device__ void mbfs_push(const MbfsView<OffT>& a, BatchStageBuf& sb, Stage& stg,
@@ -2573,12 +2547,14 @@ hipError_t launch_word_weights(const OffT* off, long long n, unsigned long long*
 hipError_t exclusive_scan_u64(const unsigned long long* in, unsigned long long* out, long long n,
                               hipStream_t stream) {
   size_t bytes = 0;
-  hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)(n + 1), stream);
+  hipError_t e = rocprim::exclusive_scan(nullptr, bytes, in, out, 0ull, (size_t)(n + 1),
+                                         rocprim::plus<unsigned long long>(), stream);
   if (e != hipSuccess) return e;
   void* tmp = nullptr;
   e = hipMalloc(&tmp, bytes);
   if (e != hipSuccess) return e;
-  e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)(n + 1), stream);
+  e = rocprim::exclusive_scan(tmp, bytes, in, out, 0ull, (size_t)(n + 1),
+                              rocprim::plus<unsigned long long>(), stream);
   const hipError_t s = hipStreamSynchronize(stream);
   (void)hipFree(tmp);
   return e != hipSuccess ? e : s;
@@ -2649,12 +2625,12 @@ hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, 
     e = hipGetLastError();
   }
   if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys, keys2, ids, order, (int)n, 0,
-                                           32, stream);
+    e = rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys2, ids, order, (size_t)n, 0, 32,
+                                  stream);
   if (e == hipSuccess) e = hipMalloc(&tmp, tmp_bytes);
   if (e == hipSuccess)
-    e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, keys2, ids, order, (int)n, 0, 32,
-                                           stream);
+    e = rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys2, ids, order, (size_t)n, 0, 32,
+                                  stream);
   if (e == hipSuccess) {
     hipLaunchKernelGGL(rank_scatter_kernel, dim3(grid), dim3(kBlock), 0, stream, order, n, rank);
     e = hipGetLastError();
@@ -2673,12 +2649,12 @@ hipError_t build_degree_ranks(const OffT* off, const int32_t* adj, long long n, 
     hipLaunchKernelGGL(nonzero_flag_kernel, dim3(grid), dim3(kBlock), 0, stream, keys, n, flags);
     e = hipGetLastError();
   }
-  hipcub::CountingInputIterator<int32_t> it(0);
+  const rocprim::counting_iterator<int32_t> it(0);
   if (e == hipSuccess)
-    e = hipcub::DeviceSelect::Flagged(nullptr, tmp2_bytes, it, flags, rows, count, (int)n, stream);
+    e = rocprim::select(nullptr, tmp2_bytes, it, flags, rows, count, (size_t)n, stream);
   if (e == hipSuccess) e = hipMalloc(&tmp2, tmp2_bytes);
   if (e == hipSuccess)
-    e = hipcub::DeviceSelect::Flagged(tmp2, tmp2_bytes, it, flags, rows, count, (int)n, stream);
+    e = rocprim::select(tmp2, tmp2_bytes, it, flags, rows, count, (size_t)n, stream);
   int32_t hcount = 0;
   if (e == hipSuccess)
     e = hipMemcpyAsync(&hcount, count, sizeof hcount, hipMemcpyDeviceToHost, stream);
@@ -2707,16 +2683,7 @@ hipError_t launch_mbfs_depth(const MbfsArgs<OffT>& a, int grid, hipStream_t stre
   return hipGetLastError();
 }
 
-#ifdef PBBS_WAVE_TIMING
-extern "C" int pbbs_debug_wave_times(unsigned long long* out, long long count) {
-  const long long cap = (long long)kTimingLevels * kTimingWaves * 3;
-  if (count > cap) count = cap;
-  return hipMemcpyFromSymbol(out, HIP_SYMBOL(g_wave_times), count * 8, 0, hipMemcpyDeviceToHost) ==
-                 hipSuccess
-             ? 0
-             : -2;
-}
-#endif
+
 
 #define PBBS_INSTANTIATE(OffT)                                                                   \
   template hipError_t launch_bfs_init<OffT>(const BfsArgs<OffT>&, BfsArgs<OffT>*,                \

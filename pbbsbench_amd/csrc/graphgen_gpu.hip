@@ -16,7 +16,8 @@
 #include <algorithm>
 #include <vector>
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
 
 #include "../../include/pbbs_bfs.h"
 #include "bfs_device.hpp"
@@ -166,12 +167,14 @@ int finish_graph(pbbs_ctx* c, int64_t n, int vb, unsigned long long* keys, int64
   GEN_TRY(bufs.alloc(&uniq, N * sizeof(unsigned long long)));
   GEN_TRY(bufs.alloc(&count, sizeof(int64_t)));
   size_t b1 = 0, b2 = 0;
-  GEN_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, b1, keys, sorted, (int64_t)N, 0, 2 * vb, st));
-  GEN_TRY(hipcub::DeviceSelect::Unique(nullptr, b2, sorted, uniq, count, (int64_t)N, st));
+  GEN_TRY(rocprim::radix_sort_keys(nullptr, b1, keys, sorted, (size_t)N, 0, 2 * vb, st));
+  GEN_TRY(rocprim::unique(nullptr, b2, sorted, uniq, count, (size_t)N,
+                          rocprim::equal_to<unsigned long long>(), st));
   void* tmp = nullptr;
   GEN_TRY(bufs.alloc(&tmp, std::max(b1, b2)));
-  GEN_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, b1, keys, sorted, (int64_t)N, 0, 2 * vb, st));
-  GEN_TRY(hipcub::DeviceSelect::Unique(tmp, b2, sorted, uniq, count, (int64_t)N, st));
+  GEN_TRY(rocprim::radix_sort_keys(tmp, b1, keys, sorted, (size_t)N, 0, 2 * vb, st));
+  GEN_TRY(rocprim::unique(tmp, b2, sorted, uniq, count, (size_t)N,
+                          rocprim::equal_to<unsigned long long>(), st));
   int64_t m = 0;
   GEN_TRY(hipMemcpyAsync(&m, count, sizeof m, hipMemcpyDeviceToHost, st));
   GEN_TRY(hipStreamSynchronize(st));
