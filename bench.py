@@ -344,7 +344,10 @@ def run_batch(args, nested):
         dg = GPU_BUILDERS["rmat22"](ctx, False)
         gen_s = time.time() - t0
         g = dg.download()
-    tuning = apply_tuning(ctx, dg, not args.no_tuning_table)
+    tuning = apply_tuning(ctx, dg, not args.no_tuning_table and not args.tune)
+    if args.tune:
+        ctx.set_tuning(**dict(args.tune))
+        tuning = {"table": None, "fields": dict(args.tune), "applied": True}
     prep_ms = dg.prepare(single=True, batch=True)
     deg = g.degrees()
     rng = np.random.default_rng(args.seed)
@@ -750,8 +753,8 @@ def main():
     ap.add_argument("--host-gen", action="store_true",
                     help="build the graphs with the host generators (default: on the GPU)")
     ap.add_argument("--tune", action="append", default=[], metavar="FIELD=VALUE",
-                    help="single: set a pbbs_tuning field (e.g. persist=0); repeatable; "
-                         "replaces the tuning table")
+                    help="single and batch: set a pbbs_tuning field (e.g. persist=0); "
+                         "repeatable; replaces the tuning table")
     args = ap.parse_args()
     tune = []
     for kv in args.tune:
