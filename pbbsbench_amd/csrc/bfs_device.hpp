@@ -94,6 +94,8 @@ struct BfsArgs {
                                // bottom-up level after top-down (num_waves + 1 entries)
   const uint32_t* bu_tops;     // [4n] row-top records {three largest neighbours or ~0,
                                // degree}, or null: bottom-up reads offsets and row tops
+  const unsigned long long* iso;  // degree-0 bitmap (nwords), set by the init kernel: a
+                               // persistent launch rebuilds visited from it and the depths
   int64_t n;
   int64_t m;
   int64_t nwords;              // ceil(n/64)

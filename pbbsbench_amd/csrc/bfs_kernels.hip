@@ -399,7 +399,7 @@ __device__ __forceinline__ void td_window(const LevelView<OffT>& a, StageBuf& sb
       }
     }
     if (__ballot(fresh)) {
-      if (fresh)
+      if (fresh && a.fb_next)   // null while a persistent launch keeps no bitmaps
         atomicOr(reinterpret_cast<unsigned*>(gen(a.fb_next)) + (v >> 5), 1u << (v & 31));
       const unsigned long long z = __ballot(fresh && deg == 0);
       if (z && lane == 0) atomicAdd(gen(a.d0_next), (unsigned)__popcll(z));
@@ -956,10 +956,64 @@ __device__ __forceinline__ int level_mode(const BfsArgs<OffT>& A, int prev, unsi
 // launch's own grid, or this BFS's part of a launch shared with others).
 // Returns false if F_k is empty (the BFS is over), true once this
 // workgroup's share of level k is done.
-template <typename OffT, bool kSorted, bool kTops>
+// A persistent launch keeps no frontier or visited bitmap while its levels
+// are top-down and their queues complete: nothing reads them then, and the
+// whole-graph clear and fold of every level is skipped, with the per-claim
+// bitmap atomicOr.  The first level that needs them (bottom-up, or a
+// frontier whose queue overflowed) first rebuilds them from the depths,
+// which every claim writes, behind one more grid barrier; the launch keeps
+// them from then on.
+struct BarView;
+__device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_ticks,
+                             BarView& b);
+struct LazyBitmaps {
+  BarView* bv;   // the launch's barrier
+  bool on;       // bitmaps not kept so far
+};
+
+// visited = {v : depth[v] < k} | degree-0 | pad bits, F_k = {v : depth[v] == k},
+// F_{k+1} = {}: the bitmaps as a level k that keeps them would find them.
+// Wavefront w of the grid builds words w*8 .. w*8+7, then every W*8-th after:
+// lane l tests vertex 64 word + l.
+template <typename OffT>
+__device__ void rebuild_bitmaps(const BfsArgs<OffT>* __restrict__ A, const int k,
+                                const unsigned vb, const unsigned vg) {
+  constexpr int kU = 8;
+  const unsigned lane = lane_id();
+  const long long W = (long long)vg * kWavesPerBlock;
+  const long long gw = (long long)vb * kWavesPerBlock + threadIdx.x / kWave;
+  const long long n = A->n, nwords = A->nwords;
+  const gptr<const int32_t> depth = glob((const int32_t*)A->depth);
+  const gptr<const unsigned long long> iso = glob(A->iso);
+  const gptr<unsigned long long> vis = glob(A->visited);
+  const gptr<unsigned long long> f0 = glob(A->fb[k % 3]);
+  const gptr<unsigned long long> f1 = glob(A->fb[(k + 1) % 3]);
+  const unsigned long long pad = (n & 63) ? (~0ull << (n & 63)) : 0ull;
+  for (long long w0 = gw * kU; w0 < nwords; w0 += W * kU) {
+    int d[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const long long v = (w0 + u) * 64 + lane;
+      d[u] = v < n ? depth[v] : INT_MAX;
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const long long w = w0 + u;
+      if (w >= nwords) break;   // wave-uniform
+      const unsigned long long lt = __ballot(d[u] < k), eq = __ballot(d[u] == k);
+      if (lane == 0) {
+        vis[w] = lt | iso[w] | (w == nwords - 1 ? pad : 0ull);
+        f0[w] = eq;
+        f1[w] = 0ull;
+      }
+    }
+  }
+}
+
+template <typename OffT, bool kSorted, bool kTops, bool kLazy = false>
 __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, const int k,
                                            StageBuf& sb, BuLds<OffT>& bl, const unsigned vb,
-                                           const unsigned vg) {
+                                           const unsigned vg, LazyBitmaps* lz = nullptr) {
   BfsState* st = A->st;
   const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
   unsigned long long cnt = 0, E = 0;
@@ -992,6 +1046,18 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
   const int prev = k == 0 ? kTopDown : st->modes[(k - 1) & 3];
   const int mode = level_mode(*A, prev, cnt, E);
   const bool qover = st->qover[c0] != 0;
+  bool keep = true;   // this level keeps the bitmaps
+  if constexpr (kLazy) {
+    if (lz->on) {
+      if (mode == kBottomUp || qover) {
+        rebuild_bitmaps(A, k, vb, vg);
+        if (!grid_barrier(st, A->host_status, A->wait_ticks, *lz->bv)) return false;
+        lz->on = false;
+      } else {
+        keep = false;
+      }
+    }
+  }
   if (vb == 0 && threadIdx.x == 0) {
     st->modes[k & 3] = mode;
     st->reached += cnt + st->d0[c0];
@@ -1010,7 +1076,7 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
     st->qover[c2] = 0;
     __hip_atomic_store(&A->host_status[1], k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  {
+  if (keep) {
     // F_{k+2}'s bitmap held F_{k-1}.  A top-down level also folds F_k into
     // the visited bitmap, which only bottom-up levels read.
     const gptr<unsigned long long> z = glob(A->fb[c2]);
@@ -1035,7 +1101,7 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
   a.depth = glob(A->depth);
   a.visited = glob(A->visited);
   a.fb_cur = glob(reinterpret_cast<const unsigned*>(A->fb[c0]));
-  a.fb_next = glob(A->fb[c1]);
+  a.fb_next = keep ? glob(A->fb[c1]) : nullptr;
   a.qcap = A->qcap;
   a.q_cur = glob(A->q[k & 1]);
   a.qe_cur = glob(A->qe[k & 1]);
@@ -1189,13 +1255,15 @@ __global__ __launch_bounds__(kBlock) void bfs_persist_kernel(const BfsArgs<OffT>
   BfsState* st = A->st;
   int k = (int)A->wg_level[blockIdx.x];
   barrier_join(blockIdx.x, gridDim.x, bv);
+  LazyBitmaps lz{&bv, true};
   {
     for (;;) {
       if (threadIdx.x == 0)
         stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
       if (stop) break;
-      if (!level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x)) break;
+      if (!level_step<OffT, kSorted, kTops, true>(A, k, sb, bl, blockIdx.x, gridDim.x, &lz))
+        break;
       ++k;
       if (!grid_barrier(st, A->host_status, A->wait_ticks, bv)) break;
     }
@@ -1268,7 +1336,9 @@ __global__ __launch_bounds__(kBlock) void bfs_init_kernel(BfsArgs<OffT> a, BfsAr
   }
   for (long long i = tid; i < kMaxGrid; i += nth) a.wg_level[i] = 0;
   if (tid == 0) {
-    *dst = a;
+    BfsArgs<OffT> b = a;
+    b.iso = iso;
+    *dst = b;
     const unsigned long long deg = (unsigned long long)(a.off[s + 1] - a.off[s]);
     BfsState* st = a.st;
     a.q[0][0] = s;

@@ -171,6 +171,36 @@ def test_edge_shapes(ctx):
     run_and_compare(ctx, g, 777)
 
 
+def test_persistent_rebuilds_bitmaps():
+    """A persistent launch keeps no frontier or visited bitmap while its levels
+    are top-down with complete queues, and rebuilds them from the depths at
+    the first level that needs them: level 0 (the hub of a star turns
+    bottom-up at once), level 1 (from a leaf of the star), level 200 (a path
+    of 200 vertices into an RMAT graph), and a level whose queue overflowed
+    (queues of 64 entries per sub-queue)."""
+    star_n = 50_001
+    star = pb.from_edges(star_n, np.zeros(star_n - 1, np.int64), np.arange(1, star_n))
+    r = pb.rmat(13, 16 * 2 ** 13, 0.57, 0.19, 0.19, seed=11)
+    lp = 200
+    ru = np.repeat(np.arange(r.n, dtype=np.int64), r.degrees()) + lp
+    rv = np.asarray(r.adj, dtype=np.int64) + lp
+    pu = np.concatenate([np.arange(lp - 1), [lp - 1], ru])
+    pv = np.concatenate([np.arange(1, lp), [lp], rv])
+    path_rmat = pb.from_edges(lp + r.n, pu, pv)
+    for tune, g, s, min_bu in [
+        (dict(persist=1), star, 0, 1),
+        (dict(persist=1), star, 777, 1),
+        (dict(persist=1), path_rmat, 0, 1),
+        (dict(persist=1, grid_blocks=7), path_rmat, 0, 1),
+        (dict(persist=1, queue_div=1_000_000, td_to_bu_div=0), path_rmat, 0, 0),
+    ]:
+        say("persistent bitmaps", tune, g.n, s)
+        with pb.Context(0) as c:
+            c.set_tuning(**tune)
+            st = run_and_compare(c, g, s)
+        assert st["launches"] == 1 and st["bu_levels"] >= min_bu, st
+
+
 def test_repeat_determinism(ctx):
     g = pb.rmat(15, 16 * 2 ** 15, 0.57, 0.19, 0.19, seed=21)
     dg = ctx.upload(g)
