@@ -432,8 +432,11 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     i = lo + (unsigned)__popcll(__ballot(p)) - 1;
   }
 
+  // The callers pass a non-empty range (every queued vertex has edges), so
+  // the first window is loaded without waiting for eb and ee; were the range
+  // empty, the window would find no edge in it and the loop would end.
   uint32_t e = eb;
-  while (e < ee) {
+  do {
     const unsigned idx = i + lane;
     int32_t qv = 0;
     uint32_t qo = 0xffffffffu;
@@ -449,16 +452,16 @@ __device__ void td_range(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     const uint32_t wend = (i + kWave < nf) ? qe[i + kWave] : E;
     const uint32_t lim = min(wend, ee);
     td_window(a, sb, stg, qv, qo, rs, e, lim);
-    e = lim;
+    e = max(e, lim);
     i += kWave;
-  }
+  } while (e < ee && i < nf);
 }
 
 // The frontier's edge space is the concatenation of the sub-queues; each
 // wavefront takes an equal slice of it.
 template <typename OffT>
 __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
-                         unsigned long long E) {
+                         unsigned long long E, const unsigned long long (&qc)[kSubQueues]) {
   const unsigned long long W = (unsigned long long)a.W;
   const unsigned long long gw = (unsigned long long)a.gw;
   if (a.small_rows) {
@@ -466,12 +469,12 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
     // within one window of balanced, and a slice's first entry is known, so
     // the search over the degree prefix is skipped.
     unsigned long long N = 0;
-    for (int s = 0; s < kSubQueues; ++s) N += (uint32_t)a.ctr_cur[s];
+    for (int s = 0; s < kSubQueues; ++s) N += (uint32_t)qc[s];
     const unsigned long long ib = N * gw / W, ie = N * (gw + 1) / W;
     if (ib >= ie) return;
     unsigned long long P = 0;
     for (int s = 0; s < kSubQueues; ++s) {
-      const unsigned long long c = a.ctr_cur[s];
+      const unsigned long long c = qc[s];
       const unsigned nf = (unsigned)c;
       const unsigned long long lo = ib > P ? ib : P;
       const unsigned long long hi = ie < P + nf ? ie : P + nf;
@@ -490,7 +493,7 @@ __device__ void top_down(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
   if (eb >= ee) return;
   unsigned long long P = 0;
   for (int s = 0; s < kSubQueues; ++s) {
-    const unsigned long long c = a.ctr_cur[s];
+    const unsigned long long c = qc[s];
     const unsigned long long Es = c >> 32;
     const unsigned long long lo = eb > P ? eb : P;
     const unsigned long long hi = ee < P + Es ? ee : P + Es;
@@ -1017,11 +1020,12 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
   BfsState* st = A->st;
   const int c0 = k % 3, c1 = (k + 1) % 3, c2 = (k + 2) % 3;
   unsigned long long cnt = 0, E = 0;
+  unsigned long long qc[kSubQueues];   // F_k's sub-queue counters, kept for the expansion
 #pragma unroll
   for (int s = 0; s < kSubQueues; ++s) {
-    const unsigned long long c = st->ctr[c0][s];
-    cnt += (uint32_t)c;
-    E += c >> 32;
+    qc[s] = st->ctr[c0][s];
+    cnt += (uint32_t)qc[s];
+    E += qc[s] >> 32;
   }
   if (cnt == 0) {
     // F_k has no vertex to expand: the BFS is over.  The first kernel to see
@@ -1130,7 +1134,7 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
     if (prev == kBottomUp || qover)
       td_from_bitmap(a, sb, stg, bl);
     else
-      top_down(a, sb, stg, E);
+      top_down(a, sb, stg, E, qc);
   } else
   {
     slot_priority(A->num_cus);   // bottom-up only: top-down levels measured the same with it
