@@ -1,9 +1,9 @@
-// Device-atomic calibration: throughput of scattered 4-byte loads and of
-// scattered atomicMax (with and without a returned value) over working sets
-// from 64 KiB to 64 MiB, and the round-trip latency of one wavefront's
-// dependent chain of loads or atomics.  The BFS claim step is one atomicMax
-// per frontier edge into an unvisited vertex, so these rates bound the levels
-// whose time is claims.
+// Device-atomic calibration: throughput of scattered 4-byte loads (plain,
+// and device-coherent: relaxed agent-scope atomic loads) and of scattered
+// atomicMax (with and without a returned value) over working sets from
+// 64 KiB to 64 MiB, and the round-trip latency of one wavefront's dependent
+// chain of each.  The BFS claim step is one atomicMax per frontier edge into
+// an unvisited vertex, so these rates bound the levels whose time is claims.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -27,7 +27,11 @@ __device__ __forceinline__ unsigned mix32(unsigned x) {
   return x;
 }
 
-enum Op { kLoad = 0, kMaxRet = 1, kMaxNoRet = 2 };
+enum Op { kLoad = 0, kMaxRet = 1, kMaxNoRet = 2, kLoadDev = 3 };
+
+__device__ __forceinline__ int load_dev(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Every thread makes `iters` accesses at scattered indices in [0, n).
 template <int kOp>
@@ -39,6 +43,7 @@ __global__ void scattered(int* a, unsigned n, int iters, int* sink) {
     if (kOp == kLoad) acc += a[idx];
     if (kOp == kMaxRet) acc += atomicMax(&a[idx], (int)t);
     if (kOp == kMaxNoRet) atomicMax(&a[idx], (int)t);
+    if (kOp == kLoadDev) acc += load_dev(&a[idx]);
   }
   if (acc == 0x7fffffff) sink[0] = acc;   // keeps the loads
 }
@@ -50,7 +55,9 @@ template <int kOp>
 __global__ void chain(int* a, unsigned n, int steps, int* sink) {
   unsigned idx = mix32(threadIdx.x) % n;
   for (int i = 0; i < steps; ++i) {
-    const int v = kOp == kLoad ? a[idx] : atomicMax(&a[idx], -1);   // -1 never changes a value
+    const int v = kOp == kLoad      ? a[idx]
+                  : kOp == kLoadDev ? load_dev(&a[idx])
+                                    : atomicMax(&a[idx], -1);   // -1 never changes a value
     idx = (unsigned)v % n;
   }
   sink[1 + threadIdx.x] = (int)idx;
@@ -70,11 +77,12 @@ int main() {
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   const int grid = 4096, block = 256, iters = 16;
-  const char* names[] = {"load", "atomicMax (returned)", "atomicMax (no return)"};
+  const char* names[] = {"load", "atomicMax (returned)", "atomicMax (no return)",
+                         "load, agent scope"};
   std::printf("scattered accesses, %d x %d threads x %d each\n", grid, block, iters);
   std::printf("| working set | op | us | G accesses/s |\n|---|---|---|---|\n");
   for (unsigned n : {16u << 10, 256u << 10, 1u << 20, 4u << 20, 16u << 20}) {
-    for (int op = 0; op < 3; ++op) {
+    for (int op = 0; op < 4; ++op) {
       CK(hipMemset(a, 0, n * sizeof(int)));
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {   // the first run warms up
@@ -82,6 +90,7 @@ int main() {
         if (op == kLoad) hipLaunchKernelGGL(scattered<kLoad>, dim3(grid), dim3(block), 0, 0, a, n, iters, sink);
         if (op == kMaxRet) hipLaunchKernelGGL(scattered<kMaxRet>, dim3(grid), dim3(block), 0, 0, a, n, iters, sink);
         if (op == kMaxNoRet) hipLaunchKernelGGL(scattered<kMaxNoRet>, dim3(grid), dim3(block), 0, 0, a, n, iters, sink);
+        if (op == kLoadDev) hipLaunchKernelGGL(scattered<kLoadDev>, dim3(grid), dim3(block), 0, 0, a, n, iters, sink);
         CK(hipGetLastError());
         CK(hipEventRecord(e1));
         CK(hipEventSynchronize(e1));
@@ -97,11 +106,12 @@ int main() {
   for (unsigned n : {16u << 10, 16u << 20}) {
     hipLaunchKernelGGL(fill_next, dim3(1024), dim3(256), 0, 0, a, n);
     CK(hipGetLastError());
-    for (int op = 0; op < 2; ++op) {
+    for (int op : {kLoad, kLoadDev, kMaxRet}) {
       float ms = 0.f;
       for (int rep = 0; rep < 2; ++rep) {
         CK(hipEventRecord(e0));
         if (op == kLoad) hipLaunchKernelGGL(chain<kLoad>, dim3(1), dim3(64), 0, 0, a, n, steps, sink);
+        else if (op == kLoadDev) hipLaunchKernelGGL(chain<kLoadDev>, dim3(1), dim3(64), 0, 0, a, n, steps, sink);
         else hipLaunchKernelGGL(chain<kMaxRet>, dim3(1), dim3(64), 0, 0, a, n, steps, sink);
         CK(hipGetLastError());
         CK(hipEventRecord(e1));
