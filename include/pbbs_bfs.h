@@ -155,6 +155,15 @@ int pbbs_graph_prepare(pbbs_ctx* ctx, pbbs_graph* g, uint32_t what, double* ms);
 
 /* ---- single-source BFS ------------------------------------------------- */
 #define PBBS_OUT_DEVICE   0x1u  /* parent/depth are device pointers (no D2H copy) */
+/* Any BFS-tree parent, the contract of backForwardBFS/BFS.C:38-61 (a
+ * compare-and-swap from -1: whichever frontier neighbour claims v first):
+ * parent[v] is some neighbour one level up, depths are exact.  Bottom-up
+ * levels then probe each row from its smallest entry up and stop at the first
+ * frontier neighbour; top-down levels still take the largest.  The outputs
+ * are the same on every run of the same inputs, pass the reference's
+ * BFSCheck (bench/BFSCheck.C:59-90), and differ from the deterministic rule
+ * only at vertices a bottom-up level reached. */
+#define PBBS_BFS_ANY_PARENT 0x4u
 
 typedef struct {
     int32_t levels;         /* BFS levels executed (max depth + 1) */

@@ -101,13 +101,16 @@ class Graph {
 
 // Parent sequence: -1 = unvisited, the start points to itself
 // (bench/BFS.h:9-11).  `depth`, if given, receives the level_number depths.
+// `flags` may hold PBBS_BFS_ANY_PARENT (backForwardBFS's contract,
+// backForwardBFS/BFS.C:38-61).
 inline std::vector<vertexId> BFS(vertexId start, const Graph& G, bool verbose = false,
                                  std::vector<vertexId>* depth = nullptr,
-                                 pbbs_bfs_stats* stats = nullptr) {
+                                 pbbs_bfs_stats* stats = nullptr, uint32_t flags = 0) {
   (void)verbose;
   std::vector<vertexId> parent(G.numVertices());
   if (depth) depth->resize(G.numVertices());
-  check(pbbs_bfs(G.ctx(), G.get(), start, parent.data(), depth ? depth->data() : nullptr, 0, stats),
+  check(pbbs_bfs(G.ctx(), G.get(), start, parent.data(), depth ? depth->data() : nullptr, flags,
+                 stats),
         "BFS");
   return parent;
 }

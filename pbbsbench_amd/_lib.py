@@ -24,6 +24,7 @@ PBBS_GRAPH_OFFSETS_I64 = 0x1
 PBBS_GRAPH_DEVICE_INPUT = 0x2
 PBBS_GRAPH_ASSUME_SYMMETRIC = 0x8
 PBBS_OUT_DEVICE = 0x1
+PBBS_BFS_ANY_PARENT = 0x4
 PBBS_PREPARE_SINGLE = 0x1
 PBBS_PREPARE_BATCH = 0x2
 

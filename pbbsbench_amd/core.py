@@ -366,13 +366,16 @@ class DeviceGraph:
         check(lib().pbbs_graph_info_get(self._h, C.byref(i)), "graph_info")
         return {name: getattr(i, name) for name, _ in i._fields_ if name != "pad"}
 
-    def bfs(self, source, depth=True):
-        """Returns (parent, depth or None, stats dict); outputs in host memory."""
+    def bfs(self, source, depth=True, any_parent=False):
+        """Returns (parent, depth or None, stats dict); outputs in host memory.
+        any_parent: the backForwardBFS contract (backForwardBFS/BFS.C:38-61),
+        any neighbour one level up as the parent; depths are the same."""
         parent = np.empty(self.n, dtype=np.int32)
         dep = np.empty(self.n, dtype=np.int32) if depth else None
         st = _lib.BfsStats()
+        flags = _lib.PBBS_BFS_ANY_PARENT if any_parent else 0
         check(lib().pbbs_bfs(self.ctx._h, self._h, int(source), _ptr(parent),
-                             _ptr(dep) if depth else None, 0, C.byref(st)), "BFS")
+                             _ptr(dep) if depth else None, flags, C.byref(st)), "BFS")
         return parent, dep, st.as_dict()
 
     def bfs_batch(self, sources, depth=True):
@@ -409,12 +412,13 @@ class DeviceGraph:
                         "frontier": r.frontier, "frontier_edges": r.frontier_edges, "ms": r.ms})
         return init.value, out
 
-    def bfs_device(self, source, parent_ptr=None, depth_ptr=None):
+    def bfs_device(self, source, parent_ptr=None, depth_ptr=None, any_parent=False):
         """BFS with outputs left in HBM (device pointers, or the context's own
         buffers when None).  Returns the stats dict."""
         st = _lib.BfsStats()
+        flags = _lib.PBBS_OUT_DEVICE | (_lib.PBBS_BFS_ANY_PARENT if any_parent else 0)
         check(lib().pbbs_bfs(self.ctx._h, self._h, int(source), parent_ptr, depth_ptr,
-                             _lib.PBBS_OUT_DEVICE, C.byref(st)), "BFS")
+                             flags, C.byref(st)), "BFS")
         return st.as_dict()
 
 

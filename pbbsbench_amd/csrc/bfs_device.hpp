@@ -67,8 +67,10 @@ struct BfsState {
 };
 
 // Single-source level kernel variants: unsorted rows (full-row bottom-up
-// scans), sorted rows read from the CSR, sorted rows with row-top records.
-enum LevelVariant { kLevelUnsorted = 0, kLevelSorted = 1, kLevelSortedTops = 2 };
+// scans), sorted rows read from the CSR, sorted rows with row-top records;
+// and any-parent levels (bottom-up rows probed from their smallest entry up,
+// the first frontier neighbour found taken as the parent, sorted or not).
+enum LevelVariant { kLevelUnsorted = 0, kLevelSorted = 1, kLevelSortedTops = 2, kLevelAnyParent = 3 };
 
 // Everything a level kernel needs.  The init kernel copies it into device
 // memory so one captured HIP graph of level launches serves every run that

@@ -542,25 +542,51 @@ __device__ __forceinline__ int scan_top(const LevelView<OffT>& a, OffT lo, OffT&
   return found;
 }
 
-// Whole-wavefront scan of the rows of the lanes in `left`, from the top, 64
-// entries per step; the first hit (lowest lane) is the highest entry.
-template <typename OffT>
+// Any-parent levels: tests entries [p, p+N) of row [p, hi) from the bottom
+// with N independent loads; returns the first hit (or -1) and moves p up
+// past them.
+template <int N, typename OffT>
+__device__ __forceinline__ int scan_bot(const LevelView<OffT>& a, OffT hi, OffT& p) {
+  const gptr<const unsigned> fbc = a.fb_cur;
+  const OffT have = hi - p;
+  int c[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    const int x = a.adj[(OffT)j < have ? p + (OffT)j : (OffT)0];
+    c[j] = (OffT)j < have ? x : -1;
+  }
+  bool h[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) h[j] = in_set_or_none(fbc, c[j]);
+  int found = -1;
+#pragma unroll
+  for (int j = N - 1; j >= 0; --j)
+    if (h[j]) found = c[j];
+  p = have > (OffT)N ? p + (OffT)N : hi;
+  return found;
+}
+
+// Whole-wavefront scan of the rows of the lanes in `left`, 64 entries per
+// step: from the top (the first hit, lowest lane, is the highest entry), or
+// with kAsc from the bottom (entries [p, bound) left; any hit is a parent).
+template <typename OffT, bool kAsc = false>
 __device__ __forceinline__ void scan_rows_together(const LevelView<OffT>& a,
-                                                   unsigned long long left, OffT lo, OffT p,
+                                                   unsigned long long left, OffT bound, OffT p,
                                                    int& found) {
   const unsigned lane = lane_id();
   const gptr<const unsigned> fbc = a.fb_cur;
   while (left) {
     const int L = __builtin_ctzll(left);
     left &= left - 1;
-    const OffT loL = shfl_off(lo, L);
+    const OffT bL = shfl_off(bound, L);
     OffT pL = shfl_off(p, L);
     int fu = -1;
-    while (pL > loL) {
-      const long long pos = (long long)pL - 1 - (long long)lane;
+    while (kAsc ? pL < bL : pL > bL) {
+      const long long pos = kAsc ? (long long)pL + (long long)lane
+                                 : (long long)pL - 1 - (long long)lane;
       bool hit = false;
       int u = -1;
-      if (pos >= (long long)loL) {
+      if (kAsc ? pos < (long long)bL : pos >= (long long)bL) {
         u = a.adj[pos];
         hit = in_set(fbc, u);
       }
@@ -569,7 +595,8 @@ __device__ __forceinline__ void scan_rows_together(const LevelView<OffT>& a,
         fu = __shfl(u, __builtin_ctzll(hm));
         break;
       }
-      pL = (pL - loL > (OffT)kWave) ? pL - (OffT)kWave : loL;
+      if (kAsc) pL = (bL - pL > (OffT)kWave) ? pL + (OffT)kWave : bL;
+      else pL = (pL - bL > (OffT)kWave) ? pL - (OffT)kWave : bL;
     }
     if (lane == (unsigned)L) found = fu;
   }
@@ -623,8 +650,9 @@ template <typename OffT>
 struct BuLds {
   int32_t q[kWavesPerBlock][kBuCap];      // unreached vertices, ascending
   int32_t lv[kWavesPerBlock][kBuCap];     // parked vertices
-  uint32_t llo[kWavesPerBlock][kBuCap];   // their row starts
-  uint32_t lp[kWavesPerBlock][kBuCap];    // their scan positions (entries [lo, p) untested)
+  uint32_t llo[kWavesPerBlock][kBuCap];   // their row starts (any-parent levels: row ends)
+  uint32_t lp[kWavesPerBlock][kBuCap];    // their scan positions (entries [lo, p) untested;
+                                          // any-parent levels: [p, end))
 };
 
 // Records the found lanes: parent/depth, their F_{k+1} bits (one atomicOr per
@@ -654,7 +682,7 @@ __device__ __forceinline__ void bu_commit(const LevelView<OffT>& a, StageBuf& sb
   stage_count(stg, nw, deg);
 }
 
-template <typename OffT>
+template <typename OffT, bool kAsc = false>
 __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                                  BuLds<OffT>& L, unsigned& npark, int tested) {
   const unsigned lane = lane_id();
@@ -663,22 +691,24 @@ __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& 
   for (unsigned base = 0; base < npark; base += kWave) {
     const bool mine = base + lane < npark;
     int v = 0;
-    OffT lo = 0, hi = 0, p = 0;
+    OffT bound = 0, p = 0;   // untested entries [bound, p) (from the top) or [p, bound)
+    uint32_t deg = 0;
     if (mine) {
       v = L.lv[w][base + lane];
-      lo = (OffT)L.llo[w][base + lane];
+      bound = (OffT)L.llo[w][base + lane];
       p = (OffT)L.lp[w][base + lane];
-      hi = p + (OffT)tested;
+      deg = kAsc ? (uint32_t)(bound - (p - (OffT)tested)) : (uint32_t)(p + (OffT)tested - bound);
     }
     int found = -1;
     // parked rows are the deep ones: sixteen loads in flight per pass
     for (int t = tested; t < a.serial_depth; t += 16) {
-      const bool active = mine && found < 0 && p > lo;
+      const bool active = mine && found < 0 && (kAsc ? p < bound : p > bound);
       if (!__ballot(active)) break;
-      if (active) found = scan_top<16>(a, lo, p);
+      if (active) found = kAsc ? scan_bot<16>(a, bound, p) : scan_top<16>(a, bound, p);
     }
-    scan_rows_together(a, __ballot(mine && found < 0 && p > lo), lo, p, found);
-    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
+    scan_rows_together<OffT, kAsc>(
+        a, __ballot(mine && found < 0 && (kAsc ? p < bound : p > bound)), bound, p, found);
+    bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, deg);
   }
   wave_lds_sync();
   npark = 0;
@@ -687,7 +717,7 @@ __device__ void bu_finish_parked(const LevelView<OffT>& a, StageBuf& sb, Stage& 
 // Parks the lanes with `park` set: vertex v with row [lo, hi), its top
 // `tested` entries already tested.  Finishes the parked rows first when they
 // would overflow the list.  Offsets fit 32 bits (m < 2^32).
-template <typename OffT>
+template <typename OffT, bool kAsc = false>
 __device__ __forceinline__ void bu_park(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                                         BuLds<OffT>& L, unsigned& npark, bool park, int v,
                                         uint32_t lo, uint32_t hi, int tested) {
@@ -695,12 +725,12 @@ __device__ __forceinline__ void bu_park(const LevelView<OffT>& a, StageBuf& sb, 
   if (!pm) return;
   const int w = threadIdx.x / kWave;
   if (npark + (unsigned)__popcll(pm) > (unsigned)kBuCap)
-    bu_finish_parked(a, sb, stg, L, npark, tested);
+    bu_finish_parked<OffT, kAsc>(a, sb, stg, L, npark, tested);
   if (park) {
     const unsigned slot = npark + lanes_below(pm);
     L.lv[w][slot] = v;
-    L.llo[w][slot] = lo;
-    L.lp[w][slot] = hi - (uint32_t)tested;
+    L.llo[w][slot] = kAsc ? hi : lo;
+    L.lp[w][slot] = kAsc ? lo + (uint32_t)tested : hi - (uint32_t)tested;
   }
   npark += (unsigned)__popcll(pm);
 }
@@ -708,7 +738,7 @@ __device__ __forceinline__ void bu_park(const LevelView<OffT>& a, StageBuf& sb, 
 // One probe pass of a sorted level over queue entries [0, cnt), cnt <=
 // kBuPer * 64: lane l takes entries l, l + 64, ...  Each row's top four
 // entries are tested; a row with no hit there and entries left is parked.
-template <typename OffT>
+template <typename OffT, bool kAsc = false>
 __device__ void bu_probe(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L,
                          unsigned cnt, unsigned& npark) {
   constexpr int N = 4;
@@ -739,7 +769,7 @@ __device__ void bu_probe(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuL
 #pragma unroll
     for (int j = 0; j < N; ++j) {
       const bool in_row = (OffT)j < hi[b] - lo[b];
-      const int x = a.adj[in_row ? hi[b] - 1 - (OffT)j : (OffT)0];
+      const int x = a.adj[in_row ? (kAsc ? lo[b] + (OffT)j : hi[b] - 1 - (OffT)j) : (OffT)0];
       c[b][j] = in_row ? x : -1;
     }
   bool h[kBuPer][N];
@@ -755,7 +785,7 @@ __device__ void bu_probe(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuL
       if (h[b][j]) found = c[b][j];
     const bool park = mine[b] && found < 0 && hi[b] - lo[b] > (OffT)N;
     bu_commit(a, sb, stg, mine[b], mine[b] && found >= 0, v[b], found, (uint32_t)(hi[b] - lo[b]));
-    bu_park(a, sb, stg, L, npark, park, v[b], (uint32_t)lo[b], (uint32_t)hi[b], N);
+    bu_park<OffT, kAsc>(a, sb, stg, L, npark, park, v[b], (uint32_t)lo[b], (uint32_t)hi[b], N);
   }
 }
 
@@ -829,10 +859,12 @@ __device__ void bu_batch_unsorted(const LevelView<OffT>& a, StageBuf& sb, Stage&
   bu_commit(a, sb, stg, mine, mine && found >= 0, v, found, (uint32_t)(hi - lo));
 }
 
-template <typename OffT, bool kSorted, bool kTops>
+template <typename OffT, bool kSorted, bool kTops, bool kAsc>
 __device__ __forceinline__ void bu_pass(const LevelView<OffT>& a, StageBuf& sb, Stage& stg,
                                         BuLds<OffT>& L, unsigned cnt, unsigned& npark) {
-  if constexpr (!kSorted)
+  if constexpr (kAsc)
+    bu_probe<OffT, true>(a, sb, stg, L, cnt, npark);
+  else if constexpr (!kSorted)
     bu_batch_unsorted(a, sb, stg, L, cnt);
   else if constexpr (kTops)
     bu_probe_tops(a, sb, stg, L, cnt, npark);
@@ -840,10 +872,12 @@ __device__ __forceinline__ void bu_pass(const LevelView<OffT>& a, StageBuf& sb, 
     bu_probe(a, sb, stg, L, cnt, npark);
 }
 
-template <typename OffT, bool kSorted, bool kTops>
+// kAsc (any-parent levels): every row is probed from its smallest entry up and
+// the first frontier neighbour found is the parent, sorted rows or not.
+template <typename OffT, bool kSorted, bool kTops, bool kAsc = false>
 __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, BuLds<OffT>& L) {
   // entries per pass: kBuPer per lane on sorted rows, one on unsorted rows
-  constexpr unsigned kPass = (unsigned)((kSorted ? kBuPer : 1) * kWave);
+  constexpr unsigned kPass = (unsigned)((kSorted || kAsc ? kBuPer : 1) * kWave);
   const unsigned lane = lane_id();
   const int w = threadIdx.x / kWave;
   const gptr<const unsigned long long> fbc64 = (gptr<const unsigned long long>)a.fb_cur;
@@ -872,7 +906,7 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
       if (take) L.q[w][nq + lanes_below(word)] = (int)((g0 + j) * kWave + lane);
       nq += c;
       if (nq >= kPass) {
-        bu_pass<OffT, kSorted, kTops>(a, sb, stg, L, kPass, npark);
+        bu_pass<OffT, kSorted, kTops, kAsc>(a, sb, stg, L, kPass, npark);
         wave_lds_sync();
         const unsigned rest = nq - kPass;   // < 64
         int tmp = lane < rest ? L.q[w][kPass + lane] : 0;
@@ -882,8 +916,9 @@ __device__ void bottom_up(const LevelView<OffT>& a, StageBuf& sb, Stage& stg, Bu
       }
     }
   }
-  if (nq) bu_pass<OffT, kSorted, kTops>(a, sb, stg, L, nq, npark);
-  if (kSorted && npark) bu_finish_parked(a, sb, stg, L, npark, kTops ? kTopEntries : 4);
+  if (nq) bu_pass<OffT, kSorted, kTops, kAsc>(a, sb, stg, L, nq, npark);
+  if ((kSorted || kAsc) && npark)
+    bu_finish_parked<OffT, kAsc>(a, sb, stg, L, npark, kTops && !kAsc ? kTopEntries : 4);
 }
 
 // A top-down level after a bottom-up one: F_k exists only as its bitmap.
@@ -1013,7 +1048,7 @@ __device__ void rebuild_bitmaps(const BfsArgs<OffT>* __restrict__ A, const int k
   }
 }
 
-template <typename OffT, bool kSorted, bool kTops, bool kLazy = false>
+template <typename OffT, bool kSorted, bool kTops, bool kLazy = false, bool kAsc = false>
 __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, const int k,
                                            StageBuf& sb, BuLds<OffT>& bl, const unsigned vb,
                                            const unsigned vg, LazyBitmaps* lz = nullptr) {
@@ -1138,7 +1173,7 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
   } else
   {
     slot_priority(A->num_cus);   // bottom-up only: top-down levels measured the same with it
-    bottom_up<OffT, kSorted, kTops>(a, sb, stg, bl);
+    bottom_up<OffT, kSorted, kTops, kAsc>(a, sb, stg, bl);
     __builtin_amdgcn_s_setprio(0);
   }
   stage_flush_block(a, sb, stg, mode == kTopDown);
@@ -1147,12 +1182,13 @@ __device__ __forceinline__ bool level_step(const BfsArgs<OffT>* __restrict__ A, 
 
 // One level per launch: the level number comes from the workgroup's own
 // counter, so a captured sequence of launches needs no host input.
-template <typename OffT, bool kSorted, bool kTops>
+template <typename OffT, bool kSorted, bool kTops, bool kAsc = false>
 __global__ __launch_bounds__(kBlock) void bfs_level_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
   const int k = (int)A->wg_level[blockIdx.x];
-  if (level_step<OffT, kSorted, kTops>(A, k, sb, bl, blockIdx.x, gridDim.x) && threadIdx.x == 0)
+  if (level_step<OffT, kSorted, kTops, false, kAsc>(A, k, sb, bl, blockIdx.x, gridDim.x) &&
+      threadIdx.x == 0)
     A->wg_level[blockIdx.x] = (uint32_t)(k + 1);
 }
 
@@ -1250,7 +1286,7 @@ __device__ bool grid_barrier(BfsState* st, int32_t* host_status, long long wait_
 // kernel's occupancy times the CU count); a launch that cannot gather at a
 // barrier within wait_ticks gives up as described above.  On leaving, each
 // workgroup records the level it reached, as the one-level kernel does.
-template <typename OffT, bool kSorted, bool kTops>
+template <typename OffT, bool kSorted, bool kTops, bool kAsc = false>
 __global__ __launch_bounds__(kBlock) void bfs_persist_kernel(const BfsArgs<OffT>* __restrict__ A) {
   __shared__ StageBuf sb;
   __shared__ BuLds<OffT> bl;
@@ -1266,7 +1302,7 @@ __global__ __launch_bounds__(kBlock) void bfs_persist_kernel(const BfsArgs<OffT>
         stop = __hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __syncthreads();
       if (stop) break;
-      if (!level_step<OffT, kSorted, kTops, true>(A, k, sb, bl, blockIdx.x, gridDim.x, &lz))
+      if (!level_step<OffT, kSorted, kTops, true, kAsc>(A, k, sb, bl, blockIdx.x, gridDim.x, &lz))
         break;
       ++k;
       if (!grid_barrier(st, A->host_status, A->wait_ticks, bv)) break;
@@ -2517,6 +2553,8 @@ hipError_t launch_bfs_init(const BfsArgs<OffT>& a, BfsArgs<OffT>* dst,
 template <typename OffT>
 const void* level_kernel_symbol(int variant) {
   switch (variant) {
+    case kLevelAnyParent:
+      return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true, false, true>);
     case kLevelSortedTops: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true, true>);
     case kLevelSorted: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, true, false>);
     default: return reinterpret_cast<const void*>(&bfs_level_kernel<OffT, false, false>);
@@ -2526,6 +2564,8 @@ const void* level_kernel_symbol(int variant) {
 template <typename OffT>
 const void* persist_kernel_symbol(int variant) {
   switch (variant) {
+    case kLevelAnyParent:
+      return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, true, false, true>);
     case kLevelSortedTops: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, true, true>);
     case kLevelSorted: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, true, false>);
     default: return reinterpret_cast<const void*>(&bfs_persist_kernel<OffT, false, false>);
@@ -2535,7 +2575,10 @@ const void* persist_kernel_symbol(int variant) {
 template <typename OffT>
 hipError_t launch_bfs_persist(const BfsArgs<OffT>* dargs, int variant, int grid,
                               hipStream_t stream) {
-  if (variant == kLevelSortedTops)
+  if (variant == kLevelAnyParent)
+    hipLaunchKernelGGL((bfs_persist_kernel<OffT, true, false, true>), dim3(grid), dim3(kBlock), 0,
+                       stream, dargs);
+  else if (variant == kLevelSortedTops)
     hipLaunchKernelGGL((bfs_persist_kernel<OffT, true, true>), dim3(grid), dim3(kBlock), 0, stream,
                        dargs);
   else if (variant == kLevelSorted)
@@ -2567,7 +2610,10 @@ hipError_t launch_bfs_multi(const BfsArgs<OffT>* args, const uint32_t* wg_begin,
 
 template <typename OffT>
 hipError_t launch_bfs_level(const BfsArgs<OffT>* dargs, int variant, int grid, hipStream_t stream) {
-  if (variant == kLevelSortedTops)
+  if (variant == kLevelAnyParent)
+    hipLaunchKernelGGL((bfs_level_kernel<OffT, true, false, true>), dim3(grid), dim3(kBlock), 0,
+                       stream, dargs);
+  else if (variant == kLevelSortedTops)
     hipLaunchKernelGGL((bfs_level_kernel<OffT, true, true>), dim3(grid), dim3(kBlock), 0, stream, dargs);
   else if (variant == kLevelSorted)
     hipLaunchKernelGGL((bfs_level_kernel<OffT, true, false>), dim3(grid), dim3(kBlock), 0, stream, dargs);

@@ -160,6 +160,7 @@ struct pbbs_ctx {
     bool finalized = false;  // batched: the depth pass after the last level is queued
     bool persist = false;    // single source: every level in one persistent launch
     bool token = false;      // holds the device's persistent-launch token
+    bool any_parent = false; // single source: PBBS_BFS_ANY_PARENT levels
     int32_t source = 0;
     int32_t* parent = nullptr;
     int32_t* depth = nullptr;
@@ -983,11 +984,11 @@ static bool want_persist(const pbbs_ctx* c, const pbbs_graph* g) {
 template <typename OffT>
 static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
                      const SingleWs& ws, bool own, bool allow_persist = true,
-                     bool restart = false) {
+                     bool restart = false, bool any_parent = false) {
   const uint32_t* tops = nullptr;
-  int rc = ensure_tops<OffT>(c, g, &tops, own);
+  int rc = any_parent ? PBBS_OK : ensure_tops<OffT>(c, g, &tops, own);
   if (rc != PBBS_OK) return rc;
-  const int variant = level_variant(g, tops);
+  const int variant = any_parent ? kLevelAnyParent : level_variant(g, tops);
   int grid = level_grid(c, g->off64, variant);
   bool persist = false;
   if (allow_persist && want_persist(c, g)) {
@@ -1017,6 +1018,7 @@ static int begin_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, i
   r.finalized = false;
   r.limit = g->n + 2;
   r.persist = persist;
+  r.any_parent = any_parent;
   r.source = s;
   r.parent = d_parent;
   r.depth = d_depth;
@@ -1048,9 +1050,10 @@ static int restart_levels(pbbs_ctx* c) {
   pbbs_ctx::Run& r = c->run;
   const SingleWs ws = r.ws;
   const bool own = r.graphs;
-  return r.off64
-             ? begin_bfs<int64_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false, true)
-             : begin_bfs<uint32_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false, true);
+  return r.off64 ? begin_bfs<int64_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false,
+                                      true, r.any_parent)
+                 : begin_bfs<uint32_t>(c, r.graph, r.source, r.parent, r.depth, ws, own, false,
+                                       true, r.any_parent);
 }
 
 static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
@@ -1071,8 +1074,9 @@ static int single_stats(pbbs_ctx* c, pbbs_bfs_stats* stats) {
 
 template <typename OffT>
 static int run_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t s, int32_t* d_parent, int32_t* d_depth,
-                   pbbs_bfs_stats* stats) {
-  int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth, own_ws(c, g->n), true);
+                   pbbs_bfs_stats* stats, bool any_parent) {
+  int rc = begin_bfs<OffT>(c, g, s, d_parent, d_depth, own_ws(c, g->n), true, true, false,
+                           any_parent);
   bool done = false;
   while (rc == PBBS_OK && !done) rc = poll_run(c, true, &done);
   if (rc != PBBS_OK) {
@@ -1092,11 +1096,14 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
   HIP_TRY(hipSetDevice(c->device));
   int rc = ensure_workspace(c, g->n);
   if (rc != PBBS_OK) return rc;
+  if (out_flags & ~(PBBS_OUT_DEVICE | PBBS_BFS_ANY_PARENT))
+    return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", out_flags);
   const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
+  const bool any = (out_flags & PBBS_BFS_ANY_PARENT) != 0;
   int32_t* dp = (dev_out && parent) ? parent : c->w_parent;
   int32_t* dd = (dev_out && depth) ? depth : c->w_depth;
-  rc = g->off64 ? run_bfs<int64_t>(c, g, source, dp, dd, stats)
-                : run_bfs<uint32_t>(c, g, source, dp, dd, stats);
+  rc = g->off64 ? run_bfs<int64_t>(c, g, source, dp, dd, stats, any)
+                : run_bfs<uint32_t>(c, g, source, dp, dd, stats, any);
   if (rc != PBBS_OK) return rc;
   if (!dev_out && (parent || depth)) {
     HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
@@ -1857,7 +1864,7 @@ static int run_multi_group(pbbs_ctx* c, const std::vector<int>& ids,
     // workgroups could not gather: run it alone
     int rc = ensure_workspace(c, gs[j]->n);
     if (rc == PBBS_OK)
-      rc = run_bfs<OffT>(c, gs[j], sources[i], dpar[i], ddep[i], &stats[i]);
+      rc = run_bfs<OffT>(c, gs[j], sources[i], dpar[i], ddep[i], &stats[i], false);
     if (rc != PBBS_OK) return rc;
   }
   return PBBS_OK;
