@@ -175,6 +175,39 @@ def traffic_from_profile(name, **match):
 
 
 # ---------------------------------------------------------------- single source
+def time_any_parent(args, dg, src, deg):
+    """The same BFS under the backForwardBFS contract (PBBS_BFS_ANY_PARENT,
+    backForwardBFS/BFS.C:38-61): device time on the bench source and on the
+    spread sources, and the GPU checker's verdict on the bench source (a valid
+    BFS tree with exact depths).  Not part of `value`."""
+    for _ in range(max(args.warmup, 1)):
+        dg.bfs_device(src, any_parent=True)
+    dev, t0 = [], time.perf_counter()
+    for _ in range(args.steps):
+        st = dg.bfs_device(src, any_parent=True)
+        dev.append(st["device_ms"])
+    wall_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    p, d, _ = dg.bfs(src, any_parent=True)
+    chk = dg.check(src, p, d)
+    out = {"contract": "any parent one level up, exact depths (backForwardBFS/BFS.C:38-61)",
+           "event_ms_per_bfs": round(float(np.mean(dev)), 4),
+           "ms_per_step": round(wall_ms, 4),
+           "gteps": round(st["edges_reached"] / (wall_ms / 1e3) / 1e9, 2),
+           "check": {"valid": chk["valid"], "depth_bad": chk["depth_bad"], "bad": chk["bad"]}}
+    if args.spread > 0:
+        rng = np.random.default_rng(args.seed + 1000)
+        others = rng.choice(np.flatnonzero(deg > 0), args.spread, replace=False)
+        sms = []
+        for s in others:
+            dg.bfs_device(int(s), any_parent=True)
+            sms.append(float(np.median([dg.bfs_device(int(s), any_parent=True)["device_ms"]
+                                        for _ in range(3)])))
+        out["spread_event_ms_mean"] = round(float(np.mean(sms)), 4)
+        out["spread_event_ms_max"] = round(float(np.max(sms)), 4)
+    say(f"single: any-parent {out['event_ms_per_bfs']} ms per BFS (device)")
+    return out
+
+
 def run_single(args):
     import pbbsbench_amd as pb
 
@@ -263,6 +296,7 @@ def run_single(args):
 
     # outputs to host once (D2H reported, not timed) and bit-exact check
     say("single: source spread done" if spread else "single: no source spread")
+    any_parent = None if args.no_any_parent else time_any_parent(args, dg, src, deg)
     parent, depth, st_host = dg.bfs(src)
     result = {
         "metric": METRIC,
@@ -297,6 +331,7 @@ def run_single(args):
             "tuning": tuning,
             "parallelism": "single GPU",
             "source_spread": spread,
+            "any_parent": any_parent,
         },
         "roofline": {
             "bound": "hbm",
@@ -738,6 +773,8 @@ def main():
                          "8 resident RMAT graphs alone; ingest: graph file (text / binary) into HBM")
     ap.add_argument("--no-batched", action="store_true", help="single: skip the batched part")
     ap.add_argument("--no-serving", action="store_true", help="single: skip the serving part")
+    ap.add_argument("--no-any-parent", action="store_true",
+                    help="single: skip timing the any-parent (backForwardBFS) contract")
     ap.add_argument("--no-multi", action="store_true",
                     help="single: skip the several-graphs-in-one-launch part")
     ap.add_argument("--no-tuning-table", action="store_true",

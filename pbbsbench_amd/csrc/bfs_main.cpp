@@ -1,13 +1,16 @@
 // BFS driver with the command line of the reference's timing driver
 // (benchmarks/breadthFirstSearch/bench/BFSTime.C:51-61):
 //
-//     BFS [-o <outFile>] [-src source] [-r <rounds>] [-v] [-d <depthFile>] [-w] <inFile>
+//     BFS [-o <outFile>] [-src source] [-r <rounds>] [-v] [-d <depthFile>] [-w] [-any] <inFile>
 //
 // It reads a PBBS AdjacencyGraph, uploads it to HBM once, runs the timing
 // protocol of common/time_loop.h:4-18 (>= 1 s of untimed warm-up runs, then
 // `rounds` timed runs, each printed as "Parlay time: <seconds>", the line
 // common/runTests.py:36 parses) and writes the parents as sequenceInt.
-// `-w` keeps int64 offsets in HBM; `-d` also writes the depth sequence.
+// `-w` keeps int64 offsets in HBM; `-d` also writes the depth sequence;
+// `-any` runs the backForwardBFS contract (backForwardBFS/BFS.C:38-61: any
+// neighbour one level up as the parent), which the reference gets by linking
+// that implementation into the same driver.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -33,7 +36,8 @@ struct Args {
     return nullptr;
   }
   [[noreturn]] void usage() const {
-    std::printf("usage: %s [-o <outFile>] [-src source] [-r <rounds>] [-v] [-d <depthFile>] [-w] <inFile>\n",
+    std::printf("usage: %s [-o <outFile>] [-src source] [-r <rounds>] [-v] [-d <depthFile>] [-w] "
+                "[-any] <inFile>\n",
                 argv[0]);
     std::exit(0);
   }
@@ -61,6 +65,7 @@ int main(int argc, char** argv) {
   const int rounds = (int)P.nonneg("-r", 1);
   const long source = P.nonneg("-src", 0);
   const bool verbose = P.has("-v");
+  const uint32_t flags = P.has("-any") ? PBBS_BFS_ANY_PARENT : 0u;
   try {
     pbbs_amd::HostGraph hg = pbbs_amd::HostGraph::read(in);
     if (source >= (long)hg.numVertices()) {
@@ -73,11 +78,12 @@ int main(int argc, char** argv) {
     pbbs_bfs_stats st{};
     // warm-up for at least one second, untimed (common/time_loop.h:8-10)
     const auto t0 = std::chrono::steady_clock::now();
-    while (seconds_since(t0) < 1.0) parents = pbbs_amd::BFS((int)source, G, verbose);
+    while (seconds_since(t0) < 1.0)
+      parents = pbbs_amd::BFS((int)source, G, verbose, nullptr, nullptr, flags);
     for (int r = 0; r < rounds; ++r) {
       parents.clear();
       const auto t = std::chrono::steady_clock::now();
-      parents = pbbs_amd::BFS((int)source, G, verbose, dout ? &depth : nullptr, &st);
+      parents = pbbs_amd::BFS((int)source, G, verbose, dout ? &depth : nullptr, &st, flags);
       std::printf("Parlay time: %.4f\n", seconds_since(t));
     }
     std::printf("\n");

@@ -65,4 +65,17 @@ def test_driver_binary_usage():
     exe = os.path.join(ROOT, "pbbsbench_amd", "bin", "BFS")
     assert os.path.exists(exe)
     out = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
-    assert "usage:" in out.stdout and "-src" in out.stdout
+    assert "usage:" in out.stdout and "-src" in out.stdout and "-any" in out.stdout
+
+
+def test_driver_reads_graph_and_rejects_bad_source():
+    """The driver parses the AdjacencyGraph file (common/graphIO.h:191-215) and
+    rejects a source outside [0, n) before it touches a device; a missing file
+    is an error too.  Neither run reaches the GPU."""
+    exe = os.path.join(ROOT, "pbbsbench_amd", "bin", "BFS")
+    g = os.path.join(ROOT, "tests", "golden", "bintree31.adj")
+    out = subprocess.run([exe, "-src", "31", g], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "source 31 outside [0, 31)" in out.stderr
+    out = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "no_such.adj")],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "readGraphFromFile" in out.stderr
