@@ -197,13 +197,17 @@ def time_any_parent(args, dg, src, deg):
     if args.spread > 0:
         rng = np.random.default_rng(args.seed + 1000)
         others = rng.choice(np.flatnonzero(deg > 0), args.spread, replace=False)
-        sms = []
+        sms, sb = [], []
+        So = 8 if dg.info["offsets_i64"] else 4
         for s in others:
             dg.bfs_device(int(s), any_parent=True)
-            sms.append(float(np.median([dg.bfs_device(int(s), any_parent=True)["device_ms"]
-                                        for _ in range(3)])))
+            rs = [dg.bfs_device(int(s), any_parent=True) for _ in range(3)]
+            sms.append(float(np.median([r["device_ms"] for r in rs])))
+            sb.append(So * (dg.n + 1) + 4 * rs[-1]["edges_reached"] + 8 * dg.n)
         out["spread_event_ms_mean"] = round(float(np.mean(sms)), 4)
         out["spread_event_ms_max"] = round(float(np.max(sms)), 4)
+        out["spread_roofline_frac_aggregate"] = round(
+            float(sum(sb)) / (float(sum(sms)) / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     say(f"single: any-parent {out['event_ms_per_bfs']} ms per BFS (device)")
     return out
 
