@@ -226,9 +226,11 @@ int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t
  * may be NULL) receive job j's outputs, n_j entries each; with
  * PBBS_OUT_DEVICE they are device pointers.  stats[j] is job j's (device_ms
  * is the whole launch's).  Every output is bit-identical to pbbs_bfs on
- * graphs[j] from sources[j].  A job whose workgroups could not gather, or
- * all jobs when another persistent launch holds the device, runs alone
- * instead, with the same outputs.  No reference counterpart: it runs the
+ * graphs[j] from sources[j].  A job on a graph of more than 2^23 edges runs
+ * alone, one launch sequence on the whole device (device_ms its own): on a
+ * share of a grid it would take longer.  A job whose workgroups could not
+ * gather, or all jobs when another persistent launch holds the device, runs
+ * alone too, with the same outputs.  No reference counterpart: it runs the
  * reference's BFS(start, G) (bench/BFS.h:11) on several graphs at once. */
 int pbbs_bfs_multi(pbbs_ctx* ctx, int32_t njobs, pbbs_graph* const* graphs,
                    const int32_t* sources, int32_t* const* parents, int32_t* const* depths,

@@ -81,6 +81,31 @@ def test_multi_tuning_invariance(tune):
         run_mix(ctx, tune)
 
 
+def test_multi_large_jobs_run_alone():
+    """Jobs on graphs of more than 2^23 edges run alone inside the call (one
+    launch sequence each on the whole device) while the small ones share one
+    launch; every job still equals the oracle and a lone pbbs_bfs."""
+    big = pb.rmat(19, 16 * 2 ** 19, 0.57, 0.19, 0.19, seed=6)
+    assert big.m > 2 ** 23
+    small = pb.rmat(12, 16 * 2 ** 12, 0.57, 0.19, 0.19, seed=2)
+    nzb = np.flatnonzero(big.degrees() > 0)
+    with pb.Context(0) as ctx:
+        db = ctx.upload(big)
+        ds = ctx.upload(small)
+        dgs = [ds, db, ds, db]
+        srcs = [0, int(nzb[0]), 7, int(nzb[9])]
+        hosts = [small, big, small, big]
+        say("large jobs alone:", [h.n for h in hosts])
+        out = ctx.bfs_multi(dgs, srcs)
+        for j, (p, d, st) in enumerate(out):
+            wp, wd = oracle.serial_bfs(hosts[j].offsets, hosts[j].adj, srcs[j])
+            assert np.array_equal(p, wp) and np.array_equal(d, wd), j
+            lp, ld, _ = dgs[j].bfs(srcs[j])
+            assert np.array_equal(p, lp) and np.array_equal(d, ld), j
+        db.free()
+        ds.free()
+
+
 def test_multi_same_graph_many_sources_and_limits():
     g = pb.rmat(12, 16 * 2 ** 12, 0.57, 0.19, 0.19, seed=9)
     with pb.Context(0) as ctx:
