@@ -209,6 +209,9 @@ int pbbs_bfs_noalloc(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* pare
  * are [nsrc][n], source-major: row i is exactly what pbbs_bfs(sources[i])
  * returns.  Needs a symmetric graph.  Stats: td_levels / bu_levels count push
  * / pull levels, reached = sum over sources of n_r, edges_reached = sum of m_r.
+ * out_flags: PBBS_OUT_DEVICE; PBBS_BFS_ANY_PARENT is accepted and changes
+ * nothing (the deterministic rows meet that contract); other bits are
+ * PBBS_ERR_INVALID, as for pbbs_bfs and pbbs_bfs_multi.
  * No reference counterpart: it batches the reference's BFS(start, G)
  * (bench/BFS.h:11) over many starts. */
 int pbbs_bfs_batch(pbbs_ctx* ctx, pbbs_graph* g, const int32_t* sources, int32_t nsrc,

@@ -1523,6 +1523,10 @@ extern "C" int pbbs_bfs_batch(pbbs_ctx* c, pbbs_graph* g, const int32_t* sources
     return fail(PBBS_ERR_INVALID,
                 "batched BFS needs a symmetric graph (parents are found by scanning each "
                 "vertex's row as its in-neighbours)");
+  // the deterministic rows also meet the any-parent contract, so that flag
+  // is accepted and changes nothing here
+  if (out_flags & ~(PBBS_OUT_DEVICE | PBBS_BFS_ANY_PARENT))
+    return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", out_flags);
   HIP_TRY(hipSetDevice(c->device));
   const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
   const int64_t entries = (int64_t)nsrc * g->n;
@@ -1914,6 +1918,9 @@ extern "C" int pbbs_bfs_multi(pbbs_ctx* c, int32_t njobs, pbbs_graph* const* gra
     c->m_host_state = hst;
     c->m_args = args;
   }
+  // as in pbbs_bfs_batch: deterministic outputs meet the any-parent contract
+  if (out_flags & ~(PBBS_OUT_DEVICE | PBBS_BFS_ANY_PARENT))
+    return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", out_flags);
   const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
   std::vector<pbbs_graph*> gs(graphs, graphs + njobs);
   // every job's workspace, one after another in m_ws, laid out before any

@@ -2,6 +2,8 @@
 
 Every case prints its name before touching the GPU so a failing run names it.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -213,12 +215,19 @@ def test_repeat_determinism(ctx):
 
 
 def test_invalid_inputs(ctx):
+    from pbbsbench_amd import _lib
     g = pb.rmat(8, 1000, 0.57, 0.19, 0.19, seed=2)
     dg = ctx.upload(g)
     with pytest.raises(pb.PbbsError):
         dg.bfs(-1)
     with pytest.raises(pb.PbbsError):
         dg.bfs(g.n)
+    # unknown flag bits are rejected by every BFS entry point
+    lib = _lib.lib()
+    assert lib.pbbs_bfs(ctx._h, dg._h, 0, None, None, 0x80, None) == _lib.PBBS_ERR_INVALID
+    src = (C.c_int32 * 1)(0)
+    assert lib.pbbs_bfs_batch(ctx._h, dg._h, src, 1, None, None, 0x80, None) == \
+        _lib.PBBS_ERR_INVALID
     dg.free()
     bad = pb.HostGraph(g.offsets, g.adj.copy())
     bad.adj[5] = g.n + 3
