@@ -113,6 +113,7 @@ def test_any_parent_shapes(ctx):
 @pytest.mark.parametrize("tune", [
     dict(persist=1),
     dict(persist=1, grid_blocks=3),
+    dict(persist=1, persist_wait_us=0),         # gives up at its first wait; runs again per level
     dict(grid_blocks=7, use_graphs=0),
     dict(bu_serial_depth=0),                    # parked rows scanned by the whole wavefront
     dict(queue_div=1_000_000),
