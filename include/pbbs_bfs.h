@@ -201,6 +201,27 @@ int pbbs_bfs(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
 int pbbs_bfs_noalloc(pbbs_ctx* ctx, pbbs_graph* g, int32_t source, int32_t* parent,
                      int32_t* depth, pbbs_bfs_stats* stats);
 
+/* ---- maximal independent set ------------------------------------------ */
+typedef struct {
+    int32_t rounds;         /* worklist rounds run (a round that found no work ends the run) */
+    int32_t pad;
+    int64_t members;        /* vertices in the set */
+    double  device_ms;      /* hipEvent time: init kernel start -> output written */
+    double  copy_ms;        /* D2H time of the flags (0 with PBBS_OUT_DEVICE) */
+} pbbs_mis_stats;
+
+/* Maximal independent set of g, the one PBBS's incrementalMIS computes
+ * (benchmarks/maximalIndependentSet/incrementalMIS/MIS.C:35-66): vertex v is
+ * in the set unless a neighbour with a lower id is -- the lexicographically
+ * first maximal independent set, the same on every run.  flags[n] (may be
+ * NULL) receives 1 for members and 2 for the others, the int sequence
+ * MISTime.C:38-43 writes; with PBBS_OUT_DEVICE it is a device pointer.  On a
+ * non-symmetric graph the rule reads each vertex's row (its out-neighbours),
+ * as the reference does.  Replaces maximalIndependentSet(G)
+ * (maximalIndependentSet/bench/MIS.h). */
+int pbbs_mis(pbbs_ctx* ctx, pbbs_graph* g, int32_t* flags, uint32_t out_flags,
+             pbbs_mis_stats* stats);
+
 /* ---- batched BFS (up to 64 sources in one traversal) ------------------- */
 #define PBBS_MAX_BATCH 64
 
@@ -378,6 +399,13 @@ int pbbs_gen_randlocal(int64_t n, int32_t k, uint64_t seed, pbbs_csr* out);
  * duplicates are always removed (common/graphUtils.h:54-61). */
 int pbbs_csr_from_edges(int64_t n, int64_t num_edges, const int32_t* u,
                         const int32_t* v, int symmetrise, pbbs_csr* out);
+
+/* Relabels g's vertices with a seeded random permutation (the one
+ * pbbs_gen_rmat's permute_seed applies) and sorts the rows: the unordered
+ * graphs the benchmarks are timed on (3Dgrid_JR, randLocalGraph_JR,
+ * rMatGraph_JR) are the ordered ones relabelled this way
+ * (common/graphUtils.h:166-189).  seed != 0. */
+int pbbs_csr_permute(const pbbs_csr* g, uint64_t seed, pbbs_csr* out);
 
 /* ---- graphs built in HBM ----------------------------------------------- */
 /* The generators below, run on the GPU: the resulting graph is bit-identical

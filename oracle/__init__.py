@@ -16,6 +16,12 @@ Parity pins (what checks each function, so that a plausible mistake fails):
   * check_bfs     accepts every oracle output; rejects planted mistakes
                   (wrong level, wrong root, unreached marked reached, cycles).
   * max_rule      zero on oracle outputs; non-zero on planted min-parents.
+  * mis           == the lexicographically first maximal independent set found
+                     by brute force over all vertex subsets of small random
+                     graphs; == closed forms (paths, stars, complete graphs,
+                     even tori: the vertices of even coordinate sum).
+  * check_mis     accepts every mis output; rejects planted mistakes (two
+                  adjacent members, an outsider with no member neighbour).
 """
 import ctypes as C
 import os
@@ -45,6 +51,9 @@ def _l():
         h.oracle_serial_bfs.argtypes = [C.c_int64, P, P, C.c_int32, P, P]
         h.oracle_max_rule.restype = C.c_int64
         h.oracle_max_rule.argtypes = [C.c_int64, P, P, C.c_int32, P, P]
+        h.oracle_mis.argtypes = [C.c_int64, P, P, P]
+        h.oracle_check_mis.argtypes = [C.c_int64, P, P, P]
+        h.oracle_check_mis.restype = C.c_int64
         _lib = h
     return _lib
 
@@ -140,3 +149,31 @@ def torus_closed_form(d):
     parent[0] = 0
     del vid
     return parent, depth.reshape(-1)
+
+
+def mis(offsets, adj):
+    """maximalIndependentSet/incrementalMIS/MIS.C:35-66: flags[v] = 1 in the
+    lexicographically first maximal independent set, 2 otherwise."""
+    off, a = _csr(offsets, adj)
+    flags = np.empty(off.size - 1, dtype=np.int32)
+    _l().oracle_mis(off.size - 1, _p(off), _p(a), _p(flags))
+    return flags
+
+
+def check_mis(offsets, adj, flags):
+    """maximalIndependentSet/bench/MISCheck.C:37-60: -1 if flags mark a maximal
+    independent set (1 = member), else the first bad vertex."""
+    off, a = _csr(offsets, adj)
+    flags = np.ascontiguousarray(flags, dtype=np.int32)
+    return int(_l().oracle_check_mis(off.size - 1, _p(off), _p(a), _p(flags)))
+
+
+def torus_mis_closed_form(d):
+    """On the d^3 torus of gridGraph.C:53-72 with d even, every edge joins
+    coordinate sums of opposite parity, and each odd vertex has a lower even
+    neighbour (one coordinate minus one): the first maximal independent set is
+    the even vertices.  Returns its flags (1 / 2)."""
+    assert d % 2 == 0
+    idx = np.arange(d)
+    i, j, k = np.meshgrid(idx, idx, idx, indexing="ij")
+    return np.where((i + j + k) % 2 == 0, 1, 2).astype(np.int32).reshape(-1)

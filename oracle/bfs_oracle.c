@@ -18,6 +18,9 @@
  *                       BFS.C:32-55 computes them), then parent[v] = the
  *                       largest u with (u,v) in E one level above v.
  *   oracle_max_rule     counts vertices violating that closed form.
+ *   oracle_mis          maximalIndependentSet/incrementalMIS/MIS.C:35-66: the
+ *                       lexicographically first maximal independent set.
+ *   oracle_check_mis    maximalIndependentSet/bench/MISCheck.C:37-60.
  *
  * Graphs are CSR with int64 offsets[n+1] and int32 adj[m].
  */
@@ -258,4 +261,41 @@ int64_t oracle_max_rule(int64_t n, const int64_t* off, const int32_t* adj, int32
         bad += parent[v] != want;
     }
     return bad;
+}
+
+/* The set incrementalMIS computes (maximalIndependentSet/incrementalMIS/
+ * MIS.C:35-66): vertex v joins unless a lower-numbered neighbour joined;
+ * its speculative rounds only reorder that work.  flags[v] = 1 in the set,
+ * 2 a lower neighbour is in it (MIS.C:37-40).  Serial, in vertex order. */
+int oracle_mis(int64_t n, const int64_t* off, const int32_t* adj, int32_t* flags) {
+    for (int64_t v = 0; v < n; ++v) {
+        int32_t f = 1;
+        for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+            const int32_t u = adj[j];
+            if (u < v && flags[u] == 1) {
+                f = 2;
+                break;
+            }
+        }
+        flags[v] = f;
+    }
+    return 0;
+}
+
+/* maximalIndependentSet/bench/MISCheck.C:37-60: no vertex of the set has a
+ * neighbour in the set, and every vertex outside it has one.  Returns the
+ * first bad vertex, or -1. */
+int64_t oracle_check_mis(int64_t n, const int64_t* off, const int32_t* adj, const int32_t* flags) {
+    for (int64_t v = 0; v < n; ++v) {
+        int has = 0;
+        for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+            const int32_t u = adj[j];
+            if (flags[u] == 1) {
+                has = 1;
+                if (flags[v] == 1) return v;
+            }
+        }
+        if (flags[v] != 1 && !has) return v;
+    }
+    return -1;
 }

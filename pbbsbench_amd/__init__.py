@@ -10,6 +10,7 @@ from .core import (  # noqa: F401
     Server,
     from_edges,
     grid3d,
+    permute,
     randlocal,
     read_graph_binary,
     read_graph_from_file,
@@ -22,6 +23,6 @@ from .core import (  # noqa: F401
 
 __all__ = [
     "BFS", "Context", "DeviceGraph", "HostGraph", "PbbsError", "Server", "from_edges", "grid3d",
-    "randlocal", "read_graph_binary", "read_graph_from_file", "read_int_seq_from_file", "rmat",
+    "permute", "randlocal", "read_graph_binary", "read_graph_from_file", "read_int_seq_from_file", "rmat",
     "write_graph_binary", "write_graph_to_file", "write_sequence_to_file",
 ]

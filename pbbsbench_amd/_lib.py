@@ -75,6 +75,19 @@ class BfsStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class MisStats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_int32),
+        ("pad", C.c_int32),
+        ("members", C.c_int64),
+        ("device_ms", C.c_double),
+        ("copy_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 class LevelProfile(C.Structure):
     _fields_ = [
         ("level", C.c_int32),
@@ -158,6 +171,7 @@ SIGNATURES = [
                                  C.POINTER(BfsStats)]),
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_noalloc", C.c_int, [_P, _P, C.c_int32, _P, _P, C.POINTER(BfsStats)]),
+    ("pbbs_mis", C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(MisStats)]),
     ("pbbs_bfs_batch", C.c_int,
      [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_profile", C.c_int,
@@ -189,6 +203,7 @@ SIGNATURES = [
     ("pbbs_gen_randlocal", C.c_int, [C.c_int64, C.c_int32, C.c_uint64, C.POINTER(Csr)]),
     ("pbbs_csr_from_edges", C.c_int,
      [C.c_int64, C.c_int64, _P, _P, C.c_int, C.POINTER(Csr)]),
+    ("pbbs_csr_permute", C.c_int, [C.POINTER(Csr), C.c_uint64, C.POINTER(Csr)]),
     ("pbbs_read_adjacency_graph", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     ("pbbs_write_adjacency_graph", C.c_int, [C.c_char_p, C.POINTER(Csr)]),
     ("pbbs_write_graph_binary", C.c_int, [C.c_char_p, C.POINTER(Csr)]),

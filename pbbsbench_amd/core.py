@@ -99,6 +99,16 @@ def rmat(scale, draws, a, b, c, seed=1, permute_seed=0):
     return HostGraph._take(out)
 
 
+def permute(g, seed):
+    """g with its vertices relabelled by a seeded random permutation, rows
+    sorted (the unordered graphs, e.g. 3Dgrid_JR and randLocalGraph_JR, are
+    written this way: common/graphUtils.h:166-189)."""
+    src = g._as_csr()
+    out = _lib.Csr()
+    check(lib().pbbs_csr_permute(C.byref(src), int(seed), C.byref(out)), "csr_permute")
+    return HostGraph._take(out)
+
+
 def grid3d(d):
     """d x d x d torus, degree 6 (testData/graphData/gridGraph.C:53-72)."""
     out = _lib.Csr()
@@ -411,6 +421,23 @@ class DeviceGraph:
             out.append({"level": r.level, "mode": "BU" if r.mode else "TD",
                         "frontier": r.frontier, "frontier_edges": r.frontier_edges, "ms": r.ms})
         return init.value, out
+
+    def mis(self):
+        """Maximal independent set (incrementalMIS/MIS.C:35-66: v is in the set
+        unless a lower-numbered neighbour is).  Returns (flags int32[n], 1 in
+        the set and 2 not, as MISTime.C writes them; stats dict)."""
+        flags = np.empty(self.n, dtype=np.int32)
+        st = _lib.MisStats()
+        check(lib().pbbs_mis(self.ctx._h, self._h, _ptr(flags), 0, C.byref(st)), "MIS")
+        return flags, st.as_dict()
+
+    def mis_device(self, flags_ptr=None):
+        """mis() with the flags left in HBM (a device pointer to n int32, or
+        the context's own buffer when None).  Returns the stats dict."""
+        st = _lib.MisStats()
+        check(lib().pbbs_mis(self.ctx._h, self._h, flags_ptr, _lib.PBBS_OUT_DEVICE,
+                             C.byref(st)), "MIS")
+        return st.as_dict()
 
     def bfs_device(self, source, parent_ptr=None, depth_ptr=None, any_parent=False):
         """BFS with outputs left in HBM (device pointers, or the context's own

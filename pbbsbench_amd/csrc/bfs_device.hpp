@@ -173,4 +173,24 @@ struct MbfsArgs {
   int32_t pad;
 };
 
+// ------------------------------------------------- maximal independent set
+// One run of pbbs_mis (mis.hip): flags[v] 0 undecided, 1 in the set, 2 out;
+// pos[v] the row position the next look at v starts from; the worklists of
+// undecided vertices and their counts, ctr[r % 3] for round r.
+template <typename OffT>
+struct MisArgs {
+  const OffT* off;
+  const int32_t* adj;
+  uint8_t* flags;              // [n]
+  uint32_t* pos;               // [n] relative to the row start
+  int32_t* list[2];            // [n] each
+  unsigned long long* ctr;     // [3]
+  uint32_t* wg_round;          // kMaxGrid per-workgroup round counters
+  int32_t* finished;
+  int32_t* host_status;        // pinned: [0] done, [1] rounds started
+  int64_t n;
+  int32_t sorted;              // rows ascending: the lower neighbours are a prefix
+  int32_t pad;
+};
+
 }  // namespace pbbs

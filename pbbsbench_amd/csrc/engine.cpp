@@ -56,6 +56,13 @@ template <typename OffT>
 hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
                              hipStream_t stream);
 template <typename OffT>
+hipError_t launch_mis_init(const MisArgs<OffT>& a, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_mis_round(const MisArgs<OffT>& a, int grid, hipStream_t stream);
+template <typename OffT>
+hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
+                             unsigned long long* members, int grid, hipStream_t stream);
+template <typename OffT>
 hipError_t launch_pull_rows(const OffT* off, const int32_t* rows, long long nrows, long long m,
                             long long waves, int32_t* out, hipStream_t stream);
 template <typename OffT>
@@ -201,6 +208,17 @@ struct pbbs_ctx {
   uint32_t* m_wg = nullptr;              // [kMaxMultiJobs + 1]
   int32_t* m_host_status = nullptr;      // pinned: 16 words per job
   BfsState* m_host_state = nullptr;      // pinned: one per job
+  // maximal independent set: per-vertex flags and row positions, the two
+  // worklists, the int32 output staged for a host copy, the control words
+  // (three round counts, the set's size, the finished flag) and the
+  // per-workgroup round counters
+  int64_t cap_sn = 0;
+  uint8_t* s_flags = nullptr;
+  uint32_t* s_pos = nullptr;
+  int32_t* s_list = nullptr;             // list0 | list1, cap_sn entries each
+  int32_t* s_out = nullptr;
+  unsigned long long* s_ctl = nullptr;   // [0..2] counts, [3] members, [4] finished
+  uint32_t* s_wg = nullptr;              // kMaxGrid
 };
 
 struct pbbs_graph {
@@ -267,6 +285,38 @@ static void free_workspace(pbbs_ctx* c) {
   c->w_qe = nullptr;
   c->cap_n = 0;
   c->cap_q = 0;
+}
+
+static void free_mis_workspace(pbbs_ctx* c) {
+  (void)hipFree(c->s_flags);
+  (void)hipFree(c->s_pos);
+  (void)hipFree(c->s_list);
+  (void)hipFree(c->s_out);
+  (void)hipFree(c->s_ctl);
+  (void)hipFree(c->s_wg);
+  c->s_flags = nullptr;
+  c->s_pos = nullptr;
+  c->s_list = c->s_out = nullptr;
+  c->s_ctl = nullptr;
+  c->s_wg = nullptr;
+  c->cap_sn = 0;
+}
+
+static int ensure_mis_workspace(pbbs_ctx* c, int64_t n) {
+  if (n <= c->cap_sn) return PBBS_OK;
+  free_mis_workspace(c);
+  if (hipMalloc(&c->s_flags, n) != hipSuccess ||
+      hipMalloc(&c->s_pos, n * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->s_list, 2 * n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->s_out, n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->s_ctl, 8 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->s_wg, kMaxGrid * sizeof(uint32_t)) != hipSuccess) {
+    (void)hipGetLastError();
+    free_mis_workspace(c);
+    return fail(PBBS_ERR_NOMEM, "set workspace allocation for n=%lld failed", (long long)n);
+  }
+  c->cap_sn = n;
+  return PBBS_OK;
 }
 
 namespace pbbs {
@@ -374,6 +424,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   for (auto& kv : c->level_graphs) (void)hipGraphExecDestroy(kv.second);
   for (auto& kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second);
   free_workspace(c);
+  free_mis_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_q);
@@ -1122,6 +1173,94 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
   return PBBS_OK;
 }
 
+// ------------------------------------------------- maximal independent set
+// Rounds run in chunks of K launches on the context's stream (16, doubling
+// to 256); after each chunk the host reads whether a round found the worklist
+// empty.  A round decides at least the lowest undecided vertex (its lower
+// neighbours were all decided before the round began), so n + 1 rounds end
+// any run; more means a fault, reported as an error.
+template <typename OffT>
+static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* stats) {
+  MisArgs<OffT> a{};
+  a.off = static_cast<const OffT*>(g->d_off);
+  a.adj = g->d_adj;
+  a.flags = c->s_flags;
+  a.pos = c->s_pos;
+  a.list[0] = c->s_list;
+  a.list[1] = c->s_list + c->cap_sn;
+  a.ctr = c->s_ctl;
+  a.wg_round = c->s_wg;
+  a.finished = reinterpret_cast<int32_t*>(c->s_ctl + 4);
+  a.host_status = c->host_status;
+  a.n = g->n;
+  a.sorted = g->sorted ? 1 : 0;
+  const int64_t need = (g->n + kBlock - 1) / kBlock;
+  const int grid = (int)std::min<int64_t>(std::max<int64_t>(need, 1), 8 * c->num_cus);
+  volatile int32_t* hs = c->host_status;
+  HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  HIP_TRY(launch_mis_init<OffT>(a, grid, c->stream));
+  const int64_t limit = g->n + 2;
+  int64_t launched = 0;
+  int K = 16;
+  for (;;) {
+    const int k = (int)std::min<int64_t>(K, limit - launched);
+    for (int i = 0; i < k; ++i) HIP_TRY(launch_mis_round<OffT>(a, grid, c->stream));
+    launched += k;
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (hs[0]) break;
+    if (launched >= limit)
+      return fail(PBBS_ERR_STATE, "set rounds did not finish in %lld launches",
+                  (long long)launched);
+    K = std::min(2 * K, 256);
+  }
+  const int32_t rounds = hs[1];
+  HIP_TRY(launch_mis_output<OffT>(a, d_out, c->s_ctl + 3, grid, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->host_status + 4, c->s_ctl + 3, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev_end));
+  if (stats) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_start, c->ev_end));
+    unsigned long long members = 0;
+    std::memcpy(&members, c->host_status + 4, sizeof(members));
+    stats->rounds = rounds;
+    stats->pad = 0;
+    stats->members = (int64_t)members;
+    stats->device_ms = ms;
+    stats->copy_ms = 0.0;
+  }
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* flags, uint32_t out_flags,
+                        pbbs_mis_stats* stats) {
+  clear_error();
+  if (!c || !g) return fail(PBBS_ERR_INVALID, "NULL context or graph");
+  if (g->device != c->device) return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  if (out_flags & ~PBBS_OUT_DEVICE) return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", out_flags);
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = ensure_mis_workspace(c, g->n);
+  if (rc != PBBS_OK) return rc;
+  const bool dev_out = (out_flags & PBBS_OUT_DEVICE) != 0;
+  int32_t* d_out = (dev_out && flags) ? flags : c->s_out;
+  rc = g->off64 ? run_mis<int64_t>(c, g, d_out, stats) : run_mis<uint32_t>(c, g, d_out, stats);
+  if (rc != PBBS_OK) return rc;
+  if (!dev_out && flags) {
+    HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
+    HIP_TRY(hipMemcpyAsync(flags, d_out, g->n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_c1));
+    if (stats) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1));
+      stats->copy_ms = ms;
+    }
+  }
+  return PBBS_OK;
+}
+
 // ------------------------------------------------------------ level profile
 template <typename OffT>
 static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
@@ -1588,6 +1727,7 @@ int64_t ctx_workspace_bytes(const pbbs_ctx* c) {
          2 * (int64_t)kSubQueues * c->cap_bn * 8;
   if (c->b_parent) b += 2 * c->cap_bout * 4;
   if (c->b_parent_alt) b += c->cap_bout * 4;
+  if (c->s_flags) b += 13 * c->cap_sn + 64 + kMaxGrid * 4;
   return b;
 }
 
@@ -1617,6 +1757,7 @@ void ctx_trim(pbbs_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   (void)hipStreamSynchronize(c->side);
   free_workspace(c);
+  free_mis_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_q);

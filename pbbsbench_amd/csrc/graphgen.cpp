@@ -215,6 +215,36 @@ int pbbs_csr_from_edges(int64_t n, int64_t num_edges, const int32_t* u, const in
   }
 }
 
+int pbbs_csr_permute(const pbbs_csr* g, uint64_t seed, pbbs_csr* out) {
+  pbbs::clear_error();
+  if (!g || !out || !g->offsets || (g->m > 0 && !g->adj))
+    return fail(PBBS_ERR_INVALID, "bad graph arguments");
+  if (seed == 0) return fail(PBBS_ERR_INVALID, "seed must be nonzero");
+  const int64_t n = g->n;
+  if (n <= 0 || n > INT32_MAX) return fail(PBBS_ERR_LIMIT, "n=%lld outside [1, 2^31)", (long long)n);
+  if (g->offsets[0] != 0 || g->offsets[n] != g->m)
+    return fail(PBBS_ERR_INVALID, "offsets do not span [0, m)");
+  for (int64_t u = 0; u < n; ++u)
+    if (g->offsets[u + 1] < g->offsets[u])
+      return fail(PBBS_ERR_INVALID, "offsets decrease at row %lld", (long long)u);
+  for (int64_t e = 0; e < g->m; ++e)
+    if (g->adj[e] < 0 || g->adj[e] >= n)
+      return fail(PBBS_ERR_INVALID, "edge %lld names vertex %d outside [0,%lld)", (long long)e,
+                  g->adj[e], (long long)n);
+  const int vb = bits_for(n);
+  try {
+    Keys keys(g->m);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t u = 0; u < n; ++u)
+      for (int64_t e = g->offsets[u]; e < g->offsets[u + 1]; ++e)
+        keys[e] = ((uint64_t)u << vb) | (uint64_t)(uint32_t)g->adj[e];
+    permute_keys(keys, n, vb, seed);
+    return csr_from_keys(n, keys, vb, out);
+  } catch (const std::bad_alloc&) {
+    return fail(PBBS_ERR_NOMEM, "edge buffer allocation failed");
+  }
+}
+
 int pbbs_gen_rmat(int32_t scale, int64_t draws, double a, double b, double c, uint64_t seed,
                   uint64_t permute_seed, pbbs_csr* out) {
   pbbs::clear_error();

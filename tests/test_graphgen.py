@@ -91,6 +91,29 @@ def test_rmat_permutation_relabels():
     assert not np.array_equal(g.degrees(), p.degrees())
 
 
+def test_permute_matches_the_generators_relabelling():
+    """permute(g, s) applies the permutation rmat's permute_seed=s applies, so
+    relabelling the ordered graph equals generating the relabelled one."""
+    g = pb.rmat(10, 8 * 1024, 0.57, 0.19, 0.19, seed=2)
+    p = pb.rmat(10, 8 * 1024, 0.57, 0.19, 0.19, seed=2, permute_seed=9)
+    q = pb.permute(g, 9)
+    assert_canonical_symmetric(q)
+    assert np.array_equal(q.offsets, p.offsets) and np.array_equal(q.adj, p.adj)
+    assert not np.array_equal(pb.permute(g, 10).adj, q.adj)
+    t = pb.permute(pb.grid3d(6), 3)
+    assert_canonical_symmetric(t)
+    assert (t.degrees() == 6).all() and t.m == 6 * 216
+
+
+def test_permute_rejects_bad_graphs():
+    g = pb.grid3d(3)
+    with pytest.raises(pb.PbbsError):
+        pb.permute(g, 0)
+    bad = pb.HostGraph(g.offsets, np.where(g.adj == 5, 27, g.adj).astype(np.int32))
+    with pytest.raises(pb.PbbsError):
+        pb.permute(bad, 1)
+
+
 def test_randlocal_distances():
     n, k = 5000, 10
     g = pb.randlocal(n, k, seed=4)

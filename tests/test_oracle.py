@@ -124,3 +124,74 @@ def test_max_rule_rejects_min_parent():
 def test_level_number_rejects_cycle():
     with pytest.raises(ValueError):
         oracle.level_number(np.array([0, 2, 1], dtype=np.int32))
+
+
+# ------------------------------------------------------------------- MIS
+def _lex_first_mis_brute(n, edges):
+    """Every subset of vertices; keep the maximal independent ones and return
+    the lexicographically first (vertex 0 in before out, then vertex 1, ...)."""
+    adj = [set() for _ in range(n)]
+    for u, v in edges:
+        if u != v:
+            adj[u].add(v)
+            adj[v].add(u)
+    best = None
+    for mask in range(1 << n):
+        S = [(mask >> i) & 1 for i in range(n)]
+        if any(S[u] and S[v] for u in range(n) for v in adj[u]):
+            continue
+        if any(not S[v] and not any(S[u] for u in adj[v]) for v in range(n)):
+            continue
+        key = tuple(1 - x for x in S)   # smaller key: earlier vertices in the set
+        if best is None or key < best[0]:
+            best = (key, S)
+    return np.array([1 if x else 2 for x in best[1]], dtype=np.int32)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_mis_equals_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 11))
+    k = int(rng.integers(0, 2 * n + 1))
+    u = rng.integers(0, n, k)
+    v = rng.integers(0, n, k)
+    g = pb.from_edges(n, u, v)
+    want = _lex_first_mis_brute(n, list(zip(u.tolist(), v.tolist())))
+    got = oracle.mis(g.offsets, g.adj)
+    assert np.array_equal(got, want), (n, list(zip(u, v)))
+    assert oracle.check_mis(g.offsets, g.adj, got) == -1
+
+
+def test_mis_closed_forms():
+    n = 101
+    path = pb.from_edges(n, np.arange(n - 1), np.arange(1, n))
+    assert np.array_equal(oracle.mis(path.offsets, path.adj),
+                          np.where(np.arange(n) % 2 == 0, 1, 2))
+    star0 = pb.from_edges(n, np.zeros(n - 1, np.int64), np.arange(1, n))
+    assert np.array_equal(oracle.mis(star0.offsets, star0.adj), np.r_[1, np.full(n - 1, 2)])
+    star_last = pb.from_edges(n, np.full(n - 1, n - 1), np.arange(n - 1))
+    assert np.array_equal(oracle.mis(star_last.offsets, star_last.adj),
+                          np.r_[np.full(n - 1, 1), 2])
+    kn = 9
+    iu, iv = np.triu_indices(kn, 1)
+    comp = pb.from_edges(kn, iu, iv)
+    assert np.array_equal(oracle.mis(comp.offsets, comp.adj), np.r_[1, np.full(kn - 1, 2)])
+    for d in (4, 6, 10):
+        t = pb.grid3d(d)
+        assert np.array_equal(oracle.mis(t.offsets, t.adj), oracle.torus_mis_closed_form(d)), d
+
+
+def test_check_mis_rejects_planted_mistakes():
+    g = pb.rmat(9, 4000, 0.57, 0.19, 0.19, seed=3)
+    f = oracle.mis(g.offsets, g.adj)
+    assert oracle.check_mis(g.offsets, g.adj, f) == -1
+    # a member's neighbour also in the set
+    v = int(np.flatnonzero((f == 1) & (g.degrees() > 0))[0])
+    u = int(g.adj[g.offsets[v]])
+    bad = f.copy()
+    bad[u] = 1
+    assert oracle.check_mis(g.offsets, g.adj, bad) >= 0
+    # a member dropped: its neighbours (or itself) lose their witness
+    bad = f.copy()
+    bad[v] = 2
+    assert oracle.check_mis(g.offsets, g.adj, bad) >= 0
