@@ -175,20 +175,28 @@ struct MbfsArgs {
 
 // ------------------------------------------------- maximal independent set
 // One run of pbbs_mis (mis.hip): flags[v] 0 undecided, 1 in the set, 2 out;
-// pos[v] the row position the next look at v starts from; the worklists of
-// undecided vertices and their counts, ctr[r % 3] for round r.
+// pos[v] the row position the next look at v starts from (written when v is
+// deferred, read in later rounds only).  A round's deferred vertices go to
+// kMisStripes regions of the next list, `region` entries each, with a count
+// per region: three sets of counts rotate (ctr + set * kMisStripes * 16, one
+// count per 128-byte line), so no one address takes every append.
+constexpr int kMisStripes = 64;
+constexpr int kMisMaxChunk = 1024;   // list entries one wavefront takes at a time
+
 template <typename OffT>
 struct MisArgs {
   const OffT* off;
   const int32_t* adj;
   uint8_t* flags;              // [n]
   uint32_t* pos;               // [n] relative to the row start
-  int32_t* list[2];            // [n] each
-  unsigned long long* ctr;     // [3]
+  int32_t* list[2];            // [kMisStripes * region] each
+  unsigned long long* ctr;     // [3][kMisStripes][16]
+  unsigned long long* part;    // [kMaxGrid] set members per output workgroup
   uint32_t* wg_round;          // kMaxGrid per-workgroup round counters
   int32_t* finished;
   int32_t* host_status;        // pinned: [0] done, [1] rounds started
   int64_t n;
+  int64_t region;              // entries per list region: ceil(n / kMisStripes) + kMisMaxChunk
   int32_t sorted;              // rows ascending: the lower neighbours are a prefix
   int32_t pad;
 };

@@ -209,15 +209,16 @@ struct pbbs_ctx {
   int32_t* m_host_status = nullptr;      // pinned: 16 words per job
   BfsState* m_host_state = nullptr;      // pinned: one per job
   // maximal independent set: per-vertex flags and row positions, the two
-  // worklists, the int32 output staged for a host copy, the control words
-  // (three round counts, the set's size, the finished flag) and the
-  // per-workgroup round counters
+  // worklists (kMisStripes regions each), the int32 output staged for a host
+  // copy, the control words (three sets of region counts, the finished flag,
+  // the set's size), per-workgroup member counts and round counters
   int64_t cap_sn = 0;
   uint8_t* s_flags = nullptr;
   uint32_t* s_pos = nullptr;
-  int32_t* s_list = nullptr;             // list0 | list1, cap_sn entries each
+  int32_t* s_list = nullptr;             // list0 | list1, kMisStripes * mis_region(cap_sn) each
   int32_t* s_out = nullptr;
-  unsigned long long* s_ctl = nullptr;   // [0..2] counts, [3] members, [4] finished
+  unsigned long long* s_ctl = nullptr;   // kMisCtlWords
+  unsigned long long* s_part = nullptr;  // kMaxGrid
   uint32_t* s_wg = nullptr;              // kMaxGrid
 };
 
@@ -287,13 +288,23 @@ static void free_workspace(pbbs_ctx* c) {
   c->cap_q = 0;
 }
 
+// Control words of a set run: three sets of kMisStripes region counts, one
+// per 128-byte line, then the finished flag and the set's size.
+constexpr int kMisCtlFinished = 3 * kMisStripes * 16;
+constexpr int kMisCtlMembers = kMisCtlFinished + 8;
+constexpr int kMisCtlWords = kMisCtlMembers + 8;
+
+static int64_t mis_region(int64_t n) { return (n + kMisStripes - 1) / kMisStripes + kMisMaxChunk; }
+
 static void free_mis_workspace(pbbs_ctx* c) {
   (void)hipFree(c->s_flags);
   (void)hipFree(c->s_pos);
   (void)hipFree(c->s_list);
   (void)hipFree(c->s_out);
   (void)hipFree(c->s_ctl);
+  (void)hipFree(c->s_part);
   (void)hipFree(c->s_wg);
+  c->s_part = nullptr;
   c->s_flags = nullptr;
   c->s_pos = nullptr;
   c->s_list = c->s_out = nullptr;
@@ -307,9 +318,10 @@ static int ensure_mis_workspace(pbbs_ctx* c, int64_t n) {
   free_mis_workspace(c);
   if (hipMalloc(&c->s_flags, n) != hipSuccess ||
       hipMalloc(&c->s_pos, n * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&c->s_list, 2 * n * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->s_list, 2 * kMisStripes * mis_region(n) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&c->s_out, n * sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&c->s_ctl, 8 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->s_ctl, kMisCtlWords * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->s_part, kMaxGrid * sizeof(unsigned long long)) != hipSuccess ||
       hipMalloc(&c->s_wg, kMaxGrid * sizeof(uint32_t)) != hipSuccess) {
     (void)hipGetLastError();
     free_mis_workspace(c);
@@ -1174,8 +1186,9 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
 }
 
 // ------------------------------------------------- maximal independent set
-// Rounds run in chunks of K launches on the context's stream (16, doubling
-// to 256); after each chunk the host reads whether a round found the worklist
+// Rounds run in chunks of K launches on the context's stream (16, then 8:
+// the relabelled benchmark graphs finish in 13-18 rounds, and a launch after
+// the last round still costs its dispatch); after each chunk the host reads whether a round found the worklist
 // empty.  A round decides at least the lowest undecided vertex (its lower
 // neighbours were all decided before the round began), so n + 1 rounds end
 // any run; more means a fault, reported as an error.
@@ -1186,11 +1199,13 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
   a.adj = g->d_adj;
   a.flags = c->s_flags;
   a.pos = c->s_pos;
+  a.region = mis_region(c->cap_sn);
   a.list[0] = c->s_list;
-  a.list[1] = c->s_list + c->cap_sn;
+  a.list[1] = c->s_list + kMisStripes * a.region;
   a.ctr = c->s_ctl;
+  a.part = c->s_part;
   a.wg_round = c->s_wg;
-  a.finished = reinterpret_cast<int32_t*>(c->s_ctl + 4);
+  a.finished = reinterpret_cast<int32_t*>(c->s_ctl + kMisCtlFinished);
   a.host_status = c->host_status;
   a.n = g->n;
   a.sorted = g->sorted ? 1 : 0;
@@ -1208,15 +1223,16 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
     launched += k;
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (hs[2]) return fail(PBBS_ERR_STATE, "set worklist region overflow");
     if (hs[0]) break;
     if (launched >= limit)
       return fail(PBBS_ERR_STATE, "set rounds did not finish in %lld launches",
                   (long long)launched);
-    K = std::min(2 * K, 256);
+    K = 8;
   }
   const int32_t rounds = hs[1];
-  HIP_TRY(launch_mis_output<OffT>(a, d_out, c->s_ctl + 3, grid, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->host_status + 4, c->s_ctl + 3, sizeof(unsigned long long),
+  HIP_TRY(launch_mis_output<OffT>(a, d_out, c->s_ctl + kMisCtlMembers, grid, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->host_status + 4, c->s_ctl + kMisCtlMembers, sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipEventRecord(c->ev_end, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev_end));
@@ -1727,7 +1743,9 @@ int64_t ctx_workspace_bytes(const pbbs_ctx* c) {
          2 * (int64_t)kSubQueues * c->cap_bn * 8;
   if (c->b_parent) b += 2 * c->cap_bout * 4;
   if (c->b_parent_alt) b += c->cap_bout * 4;
-  if (c->s_flags) b += 13 * c->cap_sn + 64 + kMaxGrid * 4;
+  if (c->s_flags)
+    b += 9 * c->cap_sn + 8 * kMisStripes * mis_region(c->cap_sn) + kMisCtlWords * 8 +
+         kMaxGrid * 12;
   return b;
 }
 

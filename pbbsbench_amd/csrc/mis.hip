@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <climits>
+
 #include "bfs_device.hpp"
 
 namespace pbbs {
@@ -27,13 +29,103 @@ __device__ __forceinline__ unsigned lanes_below_mask(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Round r reads count ctr[r % 3] and list[r & 1] (round 0: every vertex, in
-// id order, without a list), appends to list[(r + 1) & 1] and ctr[(r + 1) % 3];
-// workgroup 0 clears ctr[(r + 2) % 3], which held round r - 1's count.
+// One look at vertex v from row position p: up to kStep entries are read
+// together (their flag loads overlap).  Returns 1 (in: every lower neighbour
+// is out), 2 (out: a lower neighbour is in), 0 (wait at *p: an undecided lower
+// neighbour, or more of the row to read; *p moved past settled entries), so
+// that the caller can look again or defer v.
+constexpr int kStep = 4;
+
+template <typename OffT>
+__device__ __forceinline__ int mis_look(const MisArgs<OffT>& a, int32_t v, OffT& p, OffT hi,
+                                        bool& blocked) {
+  int32_t u[kStep];
+  uint8_t f[kStep];
+#pragma unroll
+  for (int j = 0; j < kStep; ++j) u[j] = (OffT)j < hi - p ? a.adj[p + j] : INT32_MAX;
+#pragma unroll
+  for (int j = 0; j < kStep; ++j) f[j] = u[j] < v ? a.flags[u[j]] : (uint8_t)2;
+  bool any_in = false;
+#pragma unroll
+  for (int j = 0; j < kStep; ++j) any_in |= f[j] == 1;
+  if (any_in) return 2;
+  int first_open = kStep;
+#pragma unroll
+  for (int j = kStep - 1; j >= 0; --j)
+    if (f[j] == 0) first_open = j;
+  if (first_open < kStep) {
+    p += (OffT)first_open;
+    blocked = true;
+    return 0;
+  }
+  // all kStep entries settled (out, not lower, or past the row end)
+  if (hi - p <= (OffT)kStep) return 1;
+  if (a.sorted && u[kStep - 1] >= v) return 1;   // the rest of the row is not lower
+  p += (OffT)kStep;
+  blocked = false;
+  return 0;
+}
+
+static_assert(kMisStripes == kWave, "one region count per lane of the scan");
+
+// Region s of the round's list holds entries [pre[s], pre[s + 1]) of its
+// index space; `s` is a hint no larger than the answer (indices a lane takes
+// only grow within a chunk).
+__device__ __forceinline__ int region_from(const unsigned long long* pre, unsigned long long i,
+                                           int s) {
+  while (pre[s + 1] <= i) ++s;
+  return s;
+}
+
+__device__ __forceinline__ int region_of(const unsigned long long* pre, unsigned long long i) {
+  int lo = 0, hi = kMisStripes;   // pre[lo] <= i < pre[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (pre[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Round r reads the count set r % 3 and list[r & 1] (round 0: every vertex in
+// id order, without a list), appends to list[(r + 1) & 1] and count set
+// (r + 1) % 3; workgroup 0 clears set (r + 2) % 3, which held round r - 1's
+// counts.
+//
+// Each wavefront takes chunks of consecutive list entries.  A lane holds one
+// vertex and looks at it (mis_look) until it decides or meets an undecided
+// lower neighbour; then it takes the chunk's next vertex, so a long row does
+// not hold up the other 63 lanes.  Deferred vertices collect in LDS and go to
+// the next list with one atomicAdd per chunk, on the count of the region that
+// owns the chunk: chunk k of nchunks goes to region k * kMisStripes / nchunks,
+// so the next list keeps the ids roughly ascending (lower ids first lets a
+// round see more decisions made earlier in the same round), and a region
+// receives at most ceil(nchunks / kMisStripes) chunks: at most
+// count / kMisStripes + chunk entries, within `region`.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void mis_round_kernel(MisArgs<OffT> a) {
+  __shared__ int32_t defer[kWavesPerBlock][kMisMaxChunk];
+  __shared__ unsigned long long pre[kMisStripes + 1];
   const int r = (int)a.wg_round[blockIdx.x];
-  const unsigned long long count = a.ctr[r % 3];
+  const unsigned lane = __lane_id();
+  unsigned long long count = (unsigned long long)a.n;
+  if (r > 0) {
+    if (threadIdx.x < kWave) {   // kMisStripes == kWave: one count per lane, scanned
+      const unsigned long long* cin = a.ctr + (size_t)(r % 3) * kMisStripes * 16;
+      unsigned long long c = cin[threadIdx.x * 16];
+      // a count past the region (never expected; the host reports it) is cut
+      // so that every read stays inside the list
+      if (c > (unsigned long long)a.region) c = (unsigned long long)a.region;
+      for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long t = __shfl_up(c, d);
+        if (lane >= (unsigned)d) c += t;
+      }
+      pre[threadIdx.x + 1] = c;
+      if (threadIdx.x == 0) pre[0] = 0;
+    }
+    __syncthreads();
+    count = pre[kMisStripes];
+  }
   if (count == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && !*a.finished) {
       *a.finished = 1;
@@ -43,56 +135,92 @@ __global__ __launch_bounds__(kBlock) void mis_round_kernel(MisArgs<OffT> a) {
     }
     return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.ctr[(r + 2) % 3] = 0;
-    __hip_atomic_store(&a.host_status[1], r + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (blockIdx.x == 0 && threadIdx.x < kMisStripes) {
+    a.ctr[((size_t)((r + 2) % 3) * kMisStripes + threadIdx.x) * 16] = 0;
+    if (threadIdx.x == 0)
+      __hip_atomic_store(&a.host_status[1], r + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
-  const int32_t* in = a.list[r & 1];
-  int32_t* out = a.list[(r + 1) & 1];
-  unsigned long long* out_ctr = &a.ctr[(r + 1) % 3];
-  const unsigned lane = __lane_id();
-  const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
-  // every lane of a wavefront runs the same number of iterations, so the
-  // ballot below sees the whole wavefront
-  const unsigned long long base0 = (unsigned long long)blockIdx.x * blockDim.x + (threadIdx.x - lane);
-  for (unsigned long long b = base0; b < count; b += stride) {
-    const unsigned long long i = b + lane;
-    bool wait = false;
+  // selects, not an index into the argument block (which would copy it to scratch)
+  const int32_t* in = (r & 1) ? a.list[1] : a.list[0];
+  int32_t* out = (r & 1) ? a.list[0] : a.list[1];
+  unsigned long long* cout = a.ctr + (size_t)((r + 1) % 3) * kMisStripes * 16;
+  const unsigned wib = threadIdx.x / kWave;
+  int32_t* dbuf = defer[wib];
+  const unsigned long long nwaves = (unsigned long long)gridDim.x * kWavesPerBlock;
+  const unsigned long long wave = (unsigned long long)blockIdx.x * kWavesPerBlock + wib;
+  // chunk: the list spread over every wavefront, in multiples of 64, 64..kMisMaxChunk
+  unsigned long long chunk = (count + nwaves - 1) / nwaves;
+  chunk = ((chunk + kWave - 1) / kWave) * kWave;
+  chunk = chunk < kWave ? kWave : (chunk > kMisMaxChunk ? kMisMaxChunk : chunk);
+  const unsigned long long nchunks = (count + chunk - 1) / chunk;
+  for (unsigned long long k = wave; k < nchunks; k += nwaves) {
+    const unsigned long long c0 = k * chunk;
+    const unsigned long long c1 = c0 + chunk < count ? c0 + chunk : count;
+    unsigned long long next = c0 + kWave;   // the chunk's first entry no lane holds yet
+    unsigned ndefer = 0;                    // wave-uniform
+    int s = r == 0 ? 0 : region_of(pre, c0);
+    bool have = c0 + lane < c1;
     int32_t v = 0;
-    if (i < count) {
-      v = r == 0 ? (int32_t)i : in[i];
-      const OffT lo = a.off[v], hi = a.off[v + 1];
-      OffT p = lo + (OffT)a.pos[v];
-      uint8_t f = 1;
-      for (; p < hi; ++p) {
-        const int32_t u = a.adj[p];
-        if (u >= v) {
-          if (a.sorted) break;   // the rest of the row is not lower
-          continue;
-        }
-        const uint8_t fu = a.flags[u];
-        if (fu == 1) {
-          f = 2;
-          break;
-        }
-        if (fu == 0) {
-          f = 0;
-          break;
-        }
-      }
-      if (f == 0) {
-        a.pos[v] = (uint32_t)(p - lo);
-        wait = true;
+    OffT p = 0, lo = 0, hi = 0;
+    if (have) {
+      const unsigned long long i = c0 + lane;
+      if (r == 0) {
+        v = (int32_t)i;
       } else {
-        a.flags[v] = f;
+        s = region_from(pre, i, s);
+        v = in[(unsigned long long)s * a.region + (i - pre[s])];
       }
+      lo = a.off[v];
+      hi = a.off[v + 1];
+      p = r == 0 ? lo : lo + (OffT)a.pos[v];
     }
-    const unsigned long long wm = __ballot(wait);
-    if (wm) {
+    while (__ballot(have)) {
+      int res = -1;   // -1 looking on, 0 deferred, 1 / 2 decided
+      if (have) {
+        bool blocked = false;
+        const int d = mis_look(a, v, p, hi, blocked);
+        if (d != 0) res = d;
+        else if (blocked) res = 0;
+      }
+      if (res > 0) a.flags[v] = (uint8_t)res;
+      const unsigned long long dm = __ballot(res == 0);
+      if (res == 0) {
+        a.pos[v] = (uint32_t)(p - lo);
+        dbuf[ndefer + lanes_below_mask(dm)] = v;
+      }
+      ndefer += (unsigned)__popcll(dm);
+      // lanes that finished take the chunk's next entries
+      const bool take = res >= 0 || !have;
+      const unsigned long long free_m = __ballot(take);
+      if (take) {
+        const unsigned long long i = next + lanes_below_mask(free_m);
+        have = i < c1;
+        if (have) {
+          if (r == 0) {
+            v = (int32_t)i;
+          } else {
+            s = region_from(pre, i, s);
+            v = in[(unsigned long long)s * a.region + (i - pre[s])];
+          }
+          lo = a.off[v];
+          hi = a.off[v + 1];
+          p = r == 0 ? lo : lo + (OffT)a.pos[v];
+        }
+      }
+      next += (unsigned long long)__popcll(free_m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (ndefer) {
+      const unsigned long long t = k * kMisStripes / nchunks;
       unsigned long long at = 0;
-      if (lane == 0) at = atomicAdd(out_ctr, (unsigned long long)__popcll(wm));
+      if (lane == 0) at = atomicAdd(&cout[t * 16], (unsigned long long)ndefer);
       at = __shfl(at, 0);
-      if (wait) out[at + lanes_below_mask(wm)] = v;
+      if (at + ndefer <= (unsigned long long)a.region) {
+        int32_t* dst = out + t * (unsigned long long)a.region + at;
+        for (unsigned i = lane; i < ndefer; i += kWave) dst[i] = dbuf[i];
+      } else if (lane == 0) {   // cannot happen (see above); reported, never written
+        __hip_atomic_store(&a.host_status[2], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      }
     }
   }
   if (threadIdx.x == 0) a.wg_round[blockIdx.x] = (uint32_t)(r + 1);
@@ -102,38 +230,68 @@ template <typename OffT>
 __global__ __launch_bounds__(kBlock) void mis_init_kernel(MisArgs<OffT> a) {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nth = (long long)gridDim.x * blockDim.x;
-  for (long long v = tid; v < a.n; v += nth) {
-    a.flags[v] = 0;
-    a.pos[v] = 0;
-  }
+  for (long long v = tid; v < a.n; v += nth) a.flags[v] = 0;
   for (long long i = tid; i < kMaxGrid; i += nth) a.wg_round[i] = 0;
+  for (long long i = tid; i < 3 * kMisStripes; i += nth) a.ctr[i * 16] = 0;
   if (tid == 0) {
-    a.ctr[0] = (unsigned long long)a.n;
-    a.ctr[1] = 0;
-    a.ctr[2] = 0;
-    a.ctr[3] = 0;   // the set's size, counted by the output kernel
     *a.finished = 0;
     __threadfence_system();
+    __hip_atomic_store(&a.host_status[2], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[1], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(&a.host_status[0], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 // Output in the reference's format (MISTime.C:38-43: one int per vertex, 1 in
-// the set, 2 not) and the set's size.
+// the set, 2 not), four vertices per thread, and each workgroup's count of
+// set members in part[blockIdx.x].
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void mis_output_kernel(MisArgs<OffT> a, int32_t* flags_out,
-                                                            unsigned long long* members) {
+__global__ __launch_bounds__(kBlock) void mis_output_kernel(MisArgs<OffT> a, int32_t* flags_out) {
+  __shared__ unsigned wsum[kWavesPerBlock];
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nth = (long long)gridDim.x * blockDim.x;
+  const long long n4 = a.n / 4;
+  const uchar4* f4 = reinterpret_cast<const uchar4*>(a.flags);
+  int4* o4 = reinterpret_cast<int4*>(flags_out);
+  const bool vec = (reinterpret_cast<uintptr_t>(flags_out) & 15) == 0;
   unsigned c = 0;
-  for (long long v = tid; v < a.n; v += nth) {
+  if (vec) {
+    for (long long i = tid; i < n4; i += nth) {
+      const uchar4 f = f4[i];
+      o4[i] = make_int4(f.x, f.y, f.z, f.w);
+      c += (f.x == 1) + (f.y == 1) + (f.z == 1) + (f.w == 1);
+    }
+  }
+  for (long long v = (vec ? 4 * n4 : 0) + tid; v < a.n; v += nth) {
     const uint8_t f = a.flags[v];
     flags_out[v] = f;
     c += f == 1;
   }
   for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
-  if (__lane_id() == 0 && c) atomicAdd(members, (unsigned long long)c);
+  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += wsum[w];
+    a.part[blockIdx.x] = t;
+  }
+}
+
+// The set's size: the sum of the output workgroups' counts (one workgroup).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void mis_count_kernel(MisArgs<OffT> a, int parts,
+                                                           unsigned long long* members) {
+  __shared__ unsigned long long wsum[kWavesPerBlock];
+  unsigned long long c = 0;
+  for (int i = threadIdx.x; i < parts; i += kBlock) c += a.part[i];
+  for (int d = 32; d >= 1; d >>= 1) c += __shfl_xor(c, d);
+  if (__lane_id() == 0) wsum[threadIdx.x / kWave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int w = 0; w < kWavesPerBlock; ++w) t += wsum[w];
+    *members = t;
+  }
 }
 
 }  // namespace
@@ -154,8 +312,13 @@ hipError_t launch_mis_round(const MisArgs<OffT>& a, int grid, hipStream_t stream
 template <typename OffT>
 hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
                              unsigned long long* members, int grid, hipStream_t stream) {
+  if (grid < 1 || grid > kMaxGrid) return hipErrorInvalidValue;
   hipLaunchKernelGGL((mis_output_kernel<OffT>), dim3(grid), dim3(kBlock), 0, stream, a,
-                     flags_out, members);
+                     flags_out);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((mis_count_kernel<OffT>), dim3(1), dim3(kBlock), 0, stream, a, grid,
+                     members);
   return hipGetLastError();
 }
 
