@@ -296,6 +296,23 @@ typedef struct {
 int pbbs_check_bfs(pbbs_ctx* ctx, const pbbs_graph* g, int32_t source, const int32_t* parent,
                    const int32_t* depth, uint32_t flags, pbbs_check_result* out);
 
+/* Validates a maximal independent set on the GPU, the rule of
+ * maximalIndependentSet/bench/MISCheck.C:37-63 (flags: 1 = member, anything
+ * else not): no member has a member neighbour (independent), and every
+ * non-member has one (maximal).  flags is a host array of n entries, or a
+ * device pointer with PBBS_CHECK_DEVICE_INPUT.  Replaces running
+ * bench/MISCheck on the written file. */
+typedef struct {
+    int32_t valid;          /* 1 iff both rules hold at every vertex */
+    int32_t pad;
+    int64_t first_bad;      /* smallest vertex breaking a rule, -1 if none */
+    int64_t bad_edges;      /* members with a member neighbour */
+    int64_t bad_vertices;   /* non-members with no member neighbour */
+} pbbs_mis_check_result;
+
+int pbbs_check_mis(pbbs_ctx* ctx, const pbbs_graph* g, const int32_t* flags,
+                   uint32_t check_flags, pbbs_mis_check_result* out);
+
 /* ---- serving loop ------------------------------------------------------ */
 /* Graphs stay resident; queries arrive from any host thread and complete
  * asynchronously.  A worker thread keeps up to `slots` runs in flight (each

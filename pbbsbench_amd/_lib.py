@@ -88,6 +88,19 @@ class MisStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class MisCheckResult(C.Structure):
+    _fields_ = [
+        ("valid", C.c_int32),
+        ("pad", C.c_int32),
+        ("first_bad", C.c_int64),
+        ("bad_edges", C.c_int64),
+        ("bad_vertices", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 class LevelProfile(C.Structure):
     _fields_ = [
         ("level", C.c_int32),
@@ -185,6 +198,7 @@ SIGNATURES = [
     ("pbbs_ctx_outputs", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P)]),
     ("pbbs_check_bfs", C.c_int,
      [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(CheckResult)]),
+    ("pbbs_check_mis", C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(MisCheckResult)]),
     ("pbbs_server_create", C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
     ("pbbs_server_destroy", C.c_int, [_P]),
     ("pbbs_server_add_graph", C.c_int,

@@ -431,6 +431,23 @@ class DeviceGraph:
         check(lib().pbbs_mis(self.ctx._h, self._h, _ptr(flags), 0, C.byref(st)), "MIS")
         return flags, st.as_dict()
 
+    def check_mis(self, flags):
+        """MISCheck's rule (maximalIndependentSet/bench/MISCheck.C:37-63) on the
+        GPU for host flags (int32[n]); returns the result dict."""
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        if f.size != self.n:
+            raise ValueError("flags must have n entries")
+        r = _lib.MisCheckResult()
+        check(lib().pbbs_check_mis(self.ctx._h, self._h, _ptr(f), 0, C.byref(r)), "check_mis")
+        return r.as_dict()
+
+    def check_mis_device(self, flags_ptr):
+        """check_mis() for flags already in HBM (a device pointer to n int32)."""
+        r = _lib.MisCheckResult()
+        check(lib().pbbs_check_mis(self.ctx._h, self._h, flags_ptr, _lib.PBBS_CHECK_DEVICE_INPUT,
+                                   C.byref(r)), "check_mis")
+        return r.as_dict()
+
     def mis_device(self, flags_ptr=None):
         """mis() with the flags left in HBM (a device pointer to n int32, or
         the context's own buffer when None).  Returns the stats dict."""

@@ -155,6 +155,50 @@ __global__ void check_rows_kernel(const OffT* __restrict__ off, const int32_t* _
   }
 }
 
+// Maximal-independent-set check (maximalIndependentSet/bench/MISCheck.C:37-63),
+// one wavefront per row: a member with a member neighbour is a bad edge; a
+// non-member (any flag other than 1) with no member neighbour is a bad
+// vertex.  The row scan stops at its first member neighbour unless v is a
+// member (then the whole row is read: any member neighbour is the fault).
+// ctr[0] bad edges (one per member with a member neighbour), [1] bad vertices,
+// [2] the smallest vertex of either kind.
+template <typename OffT>
+__global__ void check_mis_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ adj,
+                                 long long n, const int32_t* __restrict__ flags,
+                                 long long* __restrict__ ctr) {
+  const int lane = threadIdx.x % kWave;
+  const long long W = (long long)gridDim.x * (blockDim.x / kWave);
+  const long long w = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  long long bad_edges = 0, bad_vertices = 0, first = LLONG_MAX;
+  for (long long v = w; v < n; v += W) {
+    const bool member = flags[v] == 1;
+    const OffT lo = off[v], hi = off[v + 1];
+    bool has = false;
+    for (OffT base = lo; base < hi; base += kWave) {   // wave-uniform bounds
+      const OffT e = base + (OffT)lane;
+      const bool hit = e < hi && flags[adj[e]] == 1;
+      if (__ballot(hit)) {
+        has = true;
+        break;
+      }
+    }
+    if (lane == 0) {
+      if (member && has) {
+        ++bad_edges;
+        first = min(first, v);
+      } else if (!member && !has) {
+        ++bad_vertices;
+        first = min(first, v);
+      }
+    }
+  }
+  if (lane == 0) {
+    if (bad_edges) atomicAdd((unsigned long long*)&ctr[0], (unsigned long long)bad_edges);
+    if (bad_vertices) atomicAdd((unsigned long long*)&ctr[1], (unsigned long long)bad_vertices);
+    if (first != LLONG_MAX) atomicMin(&ctr[2], first);
+  }
+}
+
 struct DevBufs {
   std::vector<void*> p;
   ~DevBufs() {
@@ -277,6 +321,51 @@ extern "C" int pbbs_check_bfs(pbbs_ctx* c, const pbbs_graph* g, int32_t source,
   r.depth_bad = h[kDepthBad];
   r.reached = h[kReached];
   r.max_level = (int32_t)h[kMaxLevel];
+  *res = r;
+  return PBBS_OK;
+}
+
+extern "C" int pbbs_check_mis(pbbs_ctx* c, const pbbs_graph* g, const int32_t* flags,
+                              uint32_t check_flags, pbbs_mis_check_result* res) {
+  pbbs::clear_error();
+  if (!c || !g || !flags || !res) return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (check_flags & ~(uint32_t)PBBS_CHECK_DEVICE_INPUT)
+    return fail(PBBS_ERR_INVALID, "unknown check flags 0x%x", check_flags);
+  if (pbbs::graph_device(g) != pbbs::ctx_device(c))
+    return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  const int64_t n = pbbs::graph_n(g);
+  CHK_TRY(hipSetDevice(pbbs::ctx_device(c)));
+  hipStream_t st = pbbs::ctx_stream(c);
+  const int grid = std::max(1, pbbs::ctx_num_cus(c) * 8);
+  DevBufs bufs;
+  const int32_t* df = flags;
+  if (!(check_flags & PBBS_CHECK_DEVICE_INPUT)) {
+    int32_t* f2 = nullptr;
+    CHK_TRY(bufs.alloc(&f2, n * sizeof(int32_t)));
+    CHK_TRY(hipMemcpyAsync(f2, flags, n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    df = f2;
+  }
+  long long* ctr = nullptr;
+  CHK_TRY(bufs.alloc(&ctr, 3 * sizeof(long long)));
+  const long long init[3] = {0, 0, LLONG_MAX};
+  CHK_TRY(hipMemcpyAsync(ctr, init, sizeof init, hipMemcpyHostToDevice, st));
+  if (pbbs::graph_off64(g))
+    hipLaunchKernelGGL(check_mis_kernel<int64_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const int64_t*>(pbbs::graph_dev_off(g)), pbbs::graph_dev_adj(g),
+                       (long long)n, df, ctr);
+  else
+    hipLaunchKernelGGL(check_mis_kernel<uint32_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const uint32_t*>(pbbs::graph_dev_off(g)), pbbs::graph_dev_adj(g),
+                       (long long)n, df, ctr);
+  CHK_TRY(hipGetLastError());
+  long long h[3];
+  CHK_TRY(hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, st));
+  CHK_TRY(hipStreamSynchronize(st));
+  pbbs_mis_check_result r{};
+  r.bad_edges = h[0];
+  r.bad_vertices = h[1];
+  r.valid = h[0] == 0 && h[1] == 0;
+  r.first_bad = r.valid ? -1 : h[2];
   *res = r;
   return PBBS_OK;
 }

@@ -154,8 +154,56 @@ def test_mis_full_size():
         try:
             flags, st = dg.mis()
             g = dg.download()
+            r = dg.check_mis(flags)
         finally:
             dg.free()
         say(f"rounds={st['rounds']} members={st['members']} ms={st['device_ms']:.2f}")
+        assert r["valid"] == 1, r
         want = oracle.mis(g.offsets, g.adj)
         assert np.array_equal(flags, want.astype(np.int32))
+
+
+def test_check_mis_on_gpu(ctx):
+    """pbbs_check_mis accepts the set and rejects planted faults at the vertex
+    the oracle's MISCheck restatement names."""
+    g = pb.permute(pb.rmat(13, 12 * 2 ** 13, 0.55, 0.125, 0.125, seed=8), 6)
+    dg = ctx.upload(g)
+    try:
+        say("check_mis rmat13")
+        flags, _ = dg.mis()
+        r = dg.check_mis(flags)
+        assert r["valid"] == 1 and r["first_bad"] == -1
+        off = np.asarray(g.offsets)
+        members = np.flatnonzero(flags == 1)
+        # a member's neighbour joins too: bad edge at both ends
+        v = int(members[np.argmax(np.diff(off)[members] > 0)])
+        u = int(g.adj[off[v]])
+        f2 = flags.copy()
+        f2[u] = 1
+        r = dg.check_mis(f2)
+        want = oracle.check_mis(g.offsets, g.adj, f2)
+        assert r["valid"] == 0 and r["first_bad"] == want <= min(u, v)
+        assert r["bad_edges"] >= 2
+        # a member leaves: it (and maybe neighbours) have no member neighbour
+        f3 = flags.copy()
+        f3[v] = 2
+        r = dg.check_mis(f3)
+        assert r["valid"] == 0 and r["first_bad"] == oracle.check_mis(g.offsets, g.adj, f3)
+        assert r["bad_vertices"] >= 1 and r["bad_edges"] == 0
+        # the rule on device input
+        import ctypes as C
+        import os
+        hip = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib",
+                                  "libamdhip64.so"))
+        hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        hip.hipFree.argtypes = [C.c_void_p]
+        buf = C.c_void_p()
+        assert hip.hipMalloc(C.byref(buf), g.n * 4) == 0
+        try:
+            dg.mis_device(buf.value)
+            r = dg.check_mis_device(buf.value)
+            assert r["valid"] == 1
+        finally:
+            hip.hipFree(buf)
+    finally:
+        dg.free()
