@@ -759,6 +759,75 @@ def bench_ingest(args):
     print(json.dumps(result))
 
 
+MIS_GRAPH = ("rMatGraph_JR_12_16000000 (one of the three graphs the maximal independent set "
+             "benchmark times, maximalIndependentSet/bench/testInputs): 2^24 vertices, 12 * 16M "
+             "edge draws, a=.55 b=c=.125, relabelled by a seeded permutation, built in HBM")
+
+
+def run_mis(args):
+    """maximalIndependentSet (incrementalMIS/MIS.C:35-66: the lexicographically
+    first set) on MIS_GRAPH: device time per set with the flags left in HBM,
+    the GPU check of MISCheck's rule, and, with a CPU budget, the oracle's
+    serial restatement timed on the same graph and compared bit for bit."""
+    import pbbsbench_amd as pb
+    ctx = pb.Context(0)
+    t0 = time.time()
+    dg = ctx.gen_rmat(24, 12 * 16_000_000, 0.55, 0.125, 0.125, seed=1, permute_seed=2)
+    gen_s = time.time() - t0
+    n, m = dg.n, dg.m
+    for _ in range(max(args.warmup, 1)):
+        dg.mis_device()
+    dev, rounds, t0 = [], [], time.perf_counter()
+    for _ in range(args.steps):
+        st = dg.mis_device()
+        dev.append(st["device_ms"])
+        rounds.append(st["rounds"])
+    wall_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    flags, st_host = dg.mis()
+    chk = dg.check_mis(flags)
+    ev = float(np.mean(dev))
+    So = 8 if dg.info["offsets_i64"] else 4
+    B = So * (n + 1) + 4 * m + 5 * n   # offsets, rows, flags in, int32 flags out
+    out = {
+        "graph": MIS_GRAPH, "n": n, "m": m, "gen_s": round(gen_s, 3),
+        "event_ms_per_set": round(ev, 4),
+        "ms_per_step": round(wall_ms, 4),
+        "rounds_min_max": [int(min(rounds)), int(max(rounds))],
+        "members": int(st_host["members"]),
+        "d2h_ms": round(st_host["copy_ms"], 3),
+        "check": {"valid": chk["valid"], "bad_edges": chk["bad_edges"],
+                  "bad_vertices": chk["bad_vertices"]},
+        "graph_pass_gbs": round(B / (ev / 1e3) / 1e9, 1),
+        "graph_pass_frac": round(B / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+        "graph_pass_note": ("bytes of one pass over offsets, rows and flags (the set reads row "
+                            "prefixes only, so this normalises, it is not a traffic count)"),
+    }
+    exact = chk["valid"] == 1
+    if args.cpu_seconds > 0:
+        import oracle  # the checker and the CPU baseline; never the measured product
+        g = dg.download()
+        times, ref = [], None
+        t1 = time.perf_counter()
+        while len(times) < 3 and (not times or time.perf_counter() - t1 + times[-1]
+                                  < args.cpu_seconds):
+            t2 = time.perf_counter()
+            ref = oracle.mis(g.offsets, g.adj)
+            times.append(time.perf_counter() - t2)
+        same = bool(np.array_equal(ref.astype(np.int32), flags))
+        exact &= same
+        out["cpu_baseline"] = {
+            "value": round(float(np.median(times)) * 1e3, 2), "unit": "ms per set", "cores": 1,
+            "kind": "port",
+            "sample": (f"the whole set on the same graph, {len(times)} runs of the oracle's "
+                       f"serial C restatement of incrementalMIS's rule; CPU: {cpu_info()}"),
+        }
+        out["parity"] = "bit-exact flags vs oracle" if same else "MISMATCH"
+    dg.free()
+    ctx.close()
+    say(f"mis: {out['event_ms_per_set']} ms per set (device), rounds {out['rounds_min_max']}")
+    return out, exact
+
+
 def line(base, **over):
     """A stand-alone JSON line for the batch / serve modes."""
     out = {"n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
@@ -780,15 +849,19 @@ def main():
     ap.add_argument("--td-to-bu-div", type=int, default=None)
     ap.add_argument("--bu-to-td-div", type=int, default=None)
     ap.add_argument("--mode", default="single",
-                    choices=["single", "batch", "serve", "multi", "ingest"],
-                    help="single: one-source BFS with the batched and serving parts nested in "
-                         "its line (default); batch: 64 sources in one traversal vs 64 "
-                         "sequential runs on RMAT 2^22 alone; serve: Poisson query stream over "
-                         "8 resident RMAT graphs alone; ingest: graph file (text / binary) into HBM")
+                    choices=["single", "batch", "serve", "multi", "ingest", "mis"],
+                    help="single: one-source BFS with the batched, serving, multi-graph and "
+                         "independent-set parts nested in its line (default); batch: 64 sources "
+                         "in one traversal vs 64 sequential runs on RMAT 2^22 alone; serve: "
+                         "Poisson query stream over 8 resident RMAT graphs alone; ingest: graph "
+                         "file (text / binary) into HBM; mis: maximal independent set on "
+                         "rMatGraph_JR_12_16000000 alone")
     ap.add_argument("--no-batched", action="store_true", help="single: skip the batched part")
     ap.add_argument("--no-serving", action="store_true", help="single: skip the serving part")
     ap.add_argument("--no-any-parent", action="store_true",
                     help="single: skip timing the any-parent (backForwardBFS) contract")
+    ap.add_argument("--no-mis", action="store_true",
+                    help="single: skip the maximal-independent-set part")
     ap.add_argument("--no-multi", action="store_true",
                     help="single: skip the several-graphs-in-one-launch part")
     ap.add_argument("--no-tuning-table", action="store_true",
@@ -838,6 +911,16 @@ def main():
             "warmup": args.warmup, "ms_per_step": res["small16"]["ms_per_launch"],
             "higher_is_better": True, "config": res, "parity": res["parity"]})))
         sys.exit(0 if exact else 1)
+    if args.mode == "mis":
+        res, exact = run_mis(args)
+        print(json.dumps(line({
+            "metric": "maximal independent set on rMatGraph_JR_12_16000000: ms per set",
+            "value": res["event_ms_per_set"], "unit": "ms", "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+            "higher_is_better": False, "config": res,
+            "parity": res.get("parity", "GPU check only")},
+            **({"cpu_baseline": res.pop("cpu_baseline")} if "cpu_baseline" in res else {}))))
+        sys.exit(0 if exact else 1)
     if args.mode == "serve":
         res, exact = run_serve(args, nested=False)
         print(json.dumps(line({
@@ -857,6 +940,10 @@ def main():
     if not args.no_multi:
         say("multi-graph launch")
         result["multi_graph"], ok = run_multi(args)
+        exact &= ok
+    if not args.no_mis:
+        say("maximal independent set")
+        result["mis"], ok = run_mis(args)
         exact &= ok
     print(json.dumps(result))
     if not exact:

@@ -21,7 +21,7 @@ def test_bench_refuses_more_than_one_gpu():
 def test_bench_help_names_every_mode():
     r = run(os.path.join(ROOT, "bench.py"), "--help")
     assert r.returncode == 0
-    for word in ("single", "batch", "serve", "ingest", "--spread", "--workload"):
+    for word in ("single", "batch", "serve", "ingest", "mis", "--no-mis", "--spread", "--workload"):
         assert word in r.stdout
 
 
