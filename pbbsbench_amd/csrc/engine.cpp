@@ -58,7 +58,7 @@ hipError_t launch_mbfs_level(const MbfsArgs<OffT>* dargs, bool sorted, int grid,
 template <typename OffT>
 hipError_t launch_mis_init(const MisArgs<OffT>& a, int grid, hipStream_t stream);
 template <typename OffT>
-hipError_t launch_mis_round(const MisArgs<OffT>& a, int grid, hipStream_t stream);
+hipError_t launch_mis_round(const MisArgs<OffT>& a, int step, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
                              unsigned long long* members, int grid, hipStream_t stream);
@@ -1211,6 +1211,9 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
   a.sorted = g->sorted ? 1 : 0;
   const int64_t need = (g->n + kBlock - 1) / kBlock;
   const int grid = (int)std::min<int64_t>(std::max<int64_t>(need, 1), 8 * c->num_cus);
+  // row entries per look: 4 where rows average 16 entries or more, else 2
+  // (measured on the benchmark's rMat, randLocal and 3D-grid graphs)
+  const int step = g->m >= 16 * g->n ? 4 : 2;
   volatile int32_t* hs = c->host_status;
   HIP_TRY(hipEventRecord(c->ev_start, c->stream));
   HIP_TRY(launch_mis_init<OffT>(a, grid, c->stream));
@@ -1219,7 +1222,7 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
   int K = 16;
   for (;;) {
     const int k = (int)std::min<int64_t>(K, limit - launched);
-    for (int i = 0; i < k; ++i) HIP_TRY(launch_mis_round<OffT>(a, grid, c->stream));
+    for (int i = 0; i < k; ++i) HIP_TRY(launch_mis_round<OffT>(a, step, grid, c->stream));
     launched += k;
     HIP_TRY(hipEventRecord(c->ev_end, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev_end));

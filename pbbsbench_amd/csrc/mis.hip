@@ -30,13 +30,12 @@ __device__ __forceinline__ unsigned lanes_below_mask(unsigned long long m) {
 }
 
 // One look at vertex v from row position p: up to kStep entries are read
-// together (their flag loads overlap).  Returns 1 (in: every lower neighbour
+// together (their flag loads overlap; 4 on rows averaging 16 entries or more,
+// 2 on sparser graphs, where a wider look mostly reads flags it does not need).  Returns 1 (in: every lower neighbour
 // is out), 2 (out: a lower neighbour is in), 0 (wait at *p: an undecided lower
 // neighbour, or more of the row to read; *p moved past settled entries), so
 // that the caller can look again or defer v.
-constexpr int kStep = 4;
-
-template <typename OffT>
+template <int kStep, typename OffT>
 __device__ __forceinline__ int mis_look(const MisArgs<OffT>& a, int32_t v, OffT& p, OffT hi,
                                         bool& blocked) {
   int32_t u[kStep];
@@ -102,7 +101,7 @@ __device__ __forceinline__ int region_of(const unsigned long long* pre, unsigned
 // round see more decisions made earlier in the same round), and a region
 // receives at most ceil(nchunks / kMisStripes) chunks: at most
 // count / kMisStripes + chunk entries, within `region`.
-template <typename OffT>
+template <typename OffT, int kStep>
 __global__ __launch_bounds__(kBlock) void mis_round_kernel(MisArgs<OffT> a) {
   __shared__ int32_t defer[kWavesPerBlock][kMisMaxChunk];
   __shared__ unsigned long long pre[kMisStripes + 1];
@@ -178,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void mis_round_kernel(MisArgs<OffT> a) {
       int res = -1;   // -1 looking on, 0 deferred, 1 / 2 decided
       if (have) {
         bool blocked = false;
-        const int d = mis_look(a, v, p, hi, blocked);
+        const int d = mis_look<kStep>(a, v, p, hi, blocked);
         if (d != 0) res = d;
         else if (blocked) res = 0;
       }
@@ -302,10 +301,15 @@ hipError_t launch_mis_init(const MisArgs<OffT>& a, int grid, hipStream_t stream)
   return hipGetLastError();
 }
 
+// step: row entries per look, 2 or 4 (the caller picks it from the graph's
+// average degree).
 template <typename OffT>
-hipError_t launch_mis_round(const MisArgs<OffT>& a, int grid, hipStream_t stream) {
-  if (grid < 1 || grid > kMaxGrid) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((mis_round_kernel<OffT>), dim3(grid), dim3(kBlock), 0, stream, a);
+hipError_t launch_mis_round(const MisArgs<OffT>& a, int step, int grid, hipStream_t stream) {
+  if (grid < 1 || grid > kMaxGrid || (step != 2 && step != 4)) return hipErrorInvalidValue;
+  if (step == 4)
+    hipLaunchKernelGGL((mis_round_kernel<OffT, 4>), dim3(grid), dim3(kBlock), 0, stream, a);
+  else
+    hipLaunchKernelGGL((mis_round_kernel<OffT, 2>), dim3(grid), dim3(kBlock), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -324,7 +328,7 @@ hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
 
 #define PBBS_MIS_INSTANTIATE(OffT)                                                             \
   template hipError_t launch_mis_init<OffT>(const MisArgs<OffT>&, int, hipStream_t);           \
-  template hipError_t launch_mis_round<OffT>(const MisArgs<OffT>&, int, hipStream_t);          \
+  template hipError_t launch_mis_round<OffT>(const MisArgs<OffT>&, int, int, hipStream_t);     \
   template hipError_t launch_mis_output<OffT>(const MisArgs<OffT>&, int32_t*,                  \
                                               unsigned long long*, int, hipStream_t);
 PBBS_MIS_INSTANTIATE(uint32_t)
