@@ -222,6 +222,32 @@ typedef struct {
 int pbbs_mis(pbbs_ctx* ctx, pbbs_graph* g, int32_t* flags, uint32_t out_flags,
              pbbs_mis_stats* stats);
 
+/* ---- spanning forest ---------------------------------------------------- */
+#define PBBS_EDGES_DEVICE 0x2u   /* the edge arrays are device pointers */
+
+typedef struct {
+    int32_t rounds;         /* rounds of the speculative schedule */
+    int32_t pad;
+    int64_t edges;          /* forest edges */
+    double  device_ms;      /* hipEvent time: init kernel start -> forest written */
+    double  copy_ms;        /* H2D of host edges + D2H of the forest */
+} pbbs_st_stats;
+
+/* Spanning forest of the edge array (u[i], v[i]), i < m, on n vertices: the
+ * edge ids PBBS's incrementalST returns (spanningForest/incrementalST/
+ * ST.C:35-71), in its order.  That answer is the one its speculative_for
+ * schedule with granularity 100 produces (common/speculative_for.h:30-100),
+ * and the GPU runs that schedule, so the ids equal the reference's bit for
+ * bit.  forest receives *count <= n - 1 edge ids (may be NULL for the count
+ * only); with PBBS_OUT_DEVICE it is a device pointer with room for n - 1
+ * entries.  PBBS_EDGES_DEVICE: u and v are device pointers.  Edge ends must
+ * lie in [0, n).  0 < m < 300 is refused (PBBS_ERR_LIMIT): the reference's
+ * schedule starts those at round size 0 and throws.  Replaces st(E)
+ * (spanningForest/bench/ST.h:7). */
+int pbbs_spanning_forest(pbbs_ctx* ctx, int64_t n, int64_t m, const int32_t* u,
+                         const int32_t* v, uint32_t flags, int32_t* forest, int64_t* count,
+                         pbbs_st_stats* stats);
+
 /* ---- batched BFS (up to 64 sources in one traversal) ------------------- */
 #define PBBS_MAX_BATCH 64
 

@@ -54,6 +54,12 @@ def _l():
         h.oracle_mis.argtypes = [C.c_int64, P, P, P]
         h.oracle_check_mis.argtypes = [C.c_int64, P, P, P]
         h.oracle_check_mis.restype = C.c_int64
+        h.oracle_st.restype = C.c_int64
+        h.oracle_st.argtypes = [C.c_int64, C.c_int64, P, P, C.c_int64, P, P]
+        h.oracle_serial_st.restype = C.c_int64
+        h.oracle_serial_st.argtypes = [C.c_int64, C.c_int64, P, P, P]
+        h.oracle_check_st.restype = C.c_int
+        h.oracle_check_st.argtypes = [C.c_int64, C.c_int64, P, P, P, C.c_int64]
         _lib = h
     return _lib
 
@@ -177,3 +183,40 @@ def torus_mis_closed_form(d):
     idx = np.arange(d)
     i, j, k = np.meshgrid(idx, idx, idx, indexing="ij")
     return np.where((i + j + k) % 2 == 0, 1, 2).astype(np.int32).reshape(-1)
+
+
+def _edges(u, v):
+    eu = np.ascontiguousarray(u, dtype=np.int32)
+    ev = np.ascontiguousarray(v, dtype=np.int32)
+    assert eu.shape == ev.shape
+    return eu, ev
+
+
+def st(n, u, v, granularity=100):
+    """spanningForest/incrementalST/ST.C:35-71 under speculative_for's round
+    schedule (common/speculative_for.h:30-100).  Returns (forest edge ids in
+    the reference's output order, rounds), or (None, rounds) when the
+    schedule gives up as the reference does (m < 3 * granularity)."""
+    eu, ev = _edges(u, v)
+    out = np.empty(max(int(n), 1), dtype=np.int32)
+    rounds = C.c_int64()
+    k = _l().oracle_st(int(n), eu.size, _p(eu), _p(ev), int(granularity), _p(out),
+                       C.byref(rounds))
+    return (None if k < 0 else out[:k].copy()), rounds.value
+
+
+def serial_st(n, u, v):
+    """spanningForest/bench/serialST.C: the edges, in order, that join two trees."""
+    eu, ev = _edges(u, v)
+    out = np.empty(max(eu.size, 1), dtype=np.int32)
+    k = _l().oracle_serial_st(int(n), eu.size, _p(eu), _p(ev), _p(out))
+    return out[:k].copy()
+
+
+def check_st(n, u, v, forest):
+    """spanningForest/bench/STCheck.C:36-61: 0 if `forest` is a spanning forest
+    of the edges (u, v); 1 wrong size, 2 a cycle, 3 an edge id out of range."""
+    eu, ev = _edges(u, v)
+    f = np.ascontiguousarray(forest, dtype=np.int32)
+    return int(_l().oracle_check_st(int(n), eu.size, _p(eu), _p(ev), _p(f), f.size))
+

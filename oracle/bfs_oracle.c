@@ -21,6 +21,10 @@
  *   oracle_mis          maximalIndependentSet/incrementalMIS/MIS.C:35-66: the
  *                       lexicographically first maximal independent set.
  *   oracle_check_mis    maximalIndependentSet/bench/MISCheck.C:37-60.
+ *   oracle_st           spanningForest/incrementalST/ST.C:35-71 under the
+ *                       round schedule of common/speculative_for.h:30-100.
+ *   oracle_serial_st    spanningForest/bench/serialST.C (union by rank).
+ *   oracle_check_st     spanningForest/bench/STCheck.C:36-61.
  *
  * Graphs are CSR with int64 offsets[n+1] and int32 adj[m].
  */
@@ -298,4 +302,152 @@ int64_t oracle_check_mis(int64_t n, const int64_t* off, const int32_t* adj, cons
         if (flags[v] != 1 && !has) return v;
     }
     return -1;
+}
+
+/* ---- spanning forest ---------------------------------------------------- */
+
+/* Root of i: parent pointers, -1 at a root (algorithm/union_find.h:36-52;
+ * its shortcutting only shortens later finds). */
+static int32_t st_find(int32_t* parents, int32_t i) {
+    while (parents[i] >= 0) {
+        const int32_t p = parents[i];
+        if (parents[p] >= 0) parents[i] = parents[p];
+        i = p;
+    }
+    return i;
+}
+
+/* incrementalST (spanningForest/incrementalST/ST.C:35-71) run under the round
+ * schedule of speculative_for (common/speculative_for.h:30-100), serially:
+ * every round first reserves for all its iterates (edge i finds the roots
+ * u < v of its ends and write-mins i into R[v]), then commits them (edge i
+ * links v under u if R[v] still holds i); failed iterates go, in order, to
+ * the front of the next round; the round size halves or doubles with the
+ * share of failures.  The forest's edges are then R's reserved entries in
+ * vertex order (ST.C:67-69).  Returns the number of forest edges written to
+ * out (at most n), or -1 when the schedule exceeds its try limit (as the
+ * reference throws "too many iterations": with granularity 100 that is
+ * every m < 300, whose first round size is 0).  *rounds_out gets the
+ * rounds run.  Edge ids are uint (ST.h:5), so R starts at UINT_MAX. */
+int64_t oracle_st(int64_t n, int64_t m, const int32_t* eu, const int32_t* ev, int64_t granularity,
+                  int32_t* out, int64_t* rounds_out) {
+    int32_t* parents = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    uint32_t* R = (uint32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(uint32_t));
+    const int64_t max_round = m / granularity + 1;
+    int64_t* I = (int64_t*)malloc((size_t)max_round * sizeof(int64_t));
+    int64_t* hold = (int64_t*)malloc((size_t)max_round * sizeof(int64_t));
+    int32_t* su = (int32_t*)malloc((size_t)max_round * sizeof(int32_t));
+    int32_t* sv = (int32_t*)malloc((size_t)max_round * sizeof(int32_t));
+    unsigned char* keep = (unsigned char*)malloc((size_t)max_round);
+    int64_t result = 0, rounds = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        parents[i] = -1;
+        R[i] = UINT32_MAX;
+    }
+    const int64_t max_tries = 100 + 200 * granularity;
+    int64_t cur = max_round / 4, done = 0, nkeep = 0;
+    while (done < m) {
+        if (rounds++ > max_tries) {
+            result = -1;
+            break;
+        }
+        const int64_t size = cur < m - done ? cur : m - done;
+        for (int64_t i = 0; i < size; ++i) {   /* reserve */
+            I[i] = i < nkeep ? hold[i] : done + i;
+            int32_t u = st_find(parents, eu[I[i]]), v = st_find(parents, ev[I[i]]);
+            if (u > v) {
+                const int32_t t = u;
+                u = v;
+                v = t;
+            }
+            su[i] = u;
+            sv[i] = v;
+            keep[i] = u != v;
+            if (u != v && (uint32_t)I[i] < R[v]) R[v] = (uint32_t)I[i];
+        }
+        for (int64_t i = 0; i < size; ++i) {   /* commit */
+            if (!keep[i]) continue;
+            if (R[sv[i]] == (uint32_t)I[i]) {
+                parents[sv[i]] = su[i];
+                keep[i] = 0;
+            }
+        }
+        int64_t k = 0;
+        for (int64_t i = 0; i < size; ++i)
+            if (keep[i]) hold[k++] = I[i];
+        nkeep = k;
+        done += size - nkeep;
+        const float share = (float)nkeep / (float)size;   /* NaN when size == 0 */
+        if (share > .2) {
+            int64_t lo = max_round / 64 + 1;
+            if (nkeep > lo) lo = nkeep;
+            cur = cur / 2 > lo ? cur / 2 : lo;
+        } else if (share < .1) {
+            cur = cur * 2 < max_round ? cur * 2 : max_round;
+        }
+    }
+    if (result == 0) {
+        for (int64_t v = 0; v < n; ++v)
+            if (R[v] != UINT32_MAX) out[result++] = (int32_t)R[v];
+    }
+    *rounds_out = rounds;
+    free(parents);
+    free(R);
+    free(I);
+    free(hold);
+    free(su);
+    free(sv);
+    free(keep);
+    return result;
+}
+
+/* serialST (spanningForest/bench/serialST.C): edges in order, union by rank;
+ * edge i joins the forest when its ends are in different trees.  Returns the
+ * forest size; out gets the edge ids in increasing order. */
+int64_t oracle_serial_st(int64_t n, int64_t m, const int32_t* eu, const int32_t* ev, int32_t* out) {
+    int32_t* parents = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    for (int64_t i = 0; i < n; ++i) parents[i] = -1;
+    int64_t k = 0;
+    for (int64_t i = 0; i < m; ++i) {
+        int32_t u = st_find(parents, eu[i]), v = st_find(parents, ev[i]);
+        if (u == v) continue;
+        if (parents[v] < parents[u]) {   /* union_roots: higher rank (more negative) stays root */
+            const int32_t t = u;
+            u = v;
+            v = t;
+        }
+        parents[u] += parents[v];
+        parents[v] = u;
+        out[k++] = (int32_t)i;
+    }
+    free(parents);
+    return k;
+}
+
+/* spanningForest/bench/STCheck.C:36-61: the output must have as many edges
+ * as a serial spanning forest of the input, and its own edges must form a
+ * forest (a serial spanning forest of them keeps them all).  Returns 0 if
+ * so, 1 for a wrong count, 2 for a cycle (or a repeated edge), 3 for an edge
+ * id outside [0, m). */
+int oracle_check_st(int64_t n, int64_t m, const int32_t* eu, const int32_t* ev,
+                    const int32_t* forest, int64_t k) {
+    int32_t* tmp = (int32_t*)malloc((size_t)(m > 0 ? m : 1) * sizeof(int32_t));
+    const int64_t want = oracle_serial_st(n, m, eu, ev, tmp);
+    free(tmp);
+    if (k != want) return 1;
+    int32_t* parents = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    for (int64_t i = 0; i < n; ++i) parents[i] = -1;
+    int bad = 0;
+    for (int64_t j = 0; j < k && !bad; ++j) {
+        const int32_t e = forest[j];
+        if (e < 0 || e >= m) {
+            bad = 3;
+            break;
+        }
+        const int32_t u = st_find(parents, eu[e]), v = st_find(parents, ev[e]);
+        if (u == v) bad = 2;
+        else parents[u < v ? v : u] = u < v ? u : v;
+    }
+    free(parents);
+    return bad;
 }

@@ -88,6 +88,19 @@ class MisStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class StStats(C.Structure):
+    _fields_ = [
+        ("rounds", C.c_int32),
+        ("pad", C.c_int32),
+        ("edges", C.c_int64),
+        ("device_ms", C.c_double),
+        ("copy_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 class MisCheckResult(C.Structure):
     _fields_ = [
         ("valid", C.c_int32),
@@ -146,6 +159,7 @@ class ServerResult(C.Structure):
 
 
 PBBS_CHECK_DEVICE_INPUT = 0x1
+PBBS_EDGES_DEVICE = 0x2
 
 
 class CheckResult(C.Structure):
@@ -185,6 +199,9 @@ SIGNATURES = [
     ("pbbs_bfs", C.c_int, [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_noalloc", C.c_int, [_P, _P, C.c_int32, _P, _P, C.POINTER(BfsStats)]),
     ("pbbs_mis", C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(MisStats)]),
+    ("pbbs_spanning_forest", C.c_int,
+     [_P, C.c_int64, C.c_int64, _P, _P, C.c_uint32, _P, C.POINTER(C.c_int64),
+      C.POINTER(StStats)]),
     ("pbbs_bfs_batch", C.c_int,
      [_P, _P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(BfsStats)]),
     ("pbbs_bfs_profile", C.c_int,

@@ -285,6 +285,33 @@ class Context:
                                    st), "BFS multi")
         return [st[j].as_dict() for j in range(J)]
 
+    def spanning_forest(self, n, u, v):
+        """Spanning forest of the edge array (u[i], v[i]) on n vertices: the edge
+        ids incrementalST returns (spanningForest/incrementalST/ST.C:35-71), in
+        its order.  Returns (ids int32[k], stats dict)."""
+        eu = np.ascontiguousarray(u, dtype=np.int32)
+        ev = np.ascontiguousarray(v, dtype=np.int32)
+        if eu.shape != ev.shape:
+            raise ValueError("u and v differ in length")
+        out = np.empty(max(int(n) - 1, 1), dtype=np.int32)
+        cnt = C.c_int64()
+        st = _lib.StStats()
+        check(lib().pbbs_spanning_forest(self._h, int(n), eu.size, _ptr(eu), _ptr(ev), 0,
+                                         _ptr(out), C.byref(cnt), C.byref(st)), "spanning_forest")
+        return out[:cnt.value].copy(), st.as_dict()
+
+    def spanning_forest_device(self, n, m, u_ptr, v_ptr, forest_ptr=None):
+        """spanning_forest() on edge arrays already in HBM (device pointers to m
+        int32 each); the ids stay in HBM (forest_ptr, room for n - 1, or the
+        context's own buffer when None).  Returns (count, stats dict)."""
+        cnt = C.c_int64()
+        st = _lib.StStats()
+        flags = _lib.PBBS_EDGES_DEVICE | (_lib.PBBS_OUT_DEVICE if forest_ptr else 0)
+        check(lib().pbbs_spanning_forest(self._h, int(n), int(m), u_ptr, v_ptr, flags,
+                                         forest_ptr, C.byref(cnt), C.byref(st)),
+              "spanning_forest")
+        return cnt.value, st.as_dict()
+
     def gen_rmat(self, scale, draws, a, b, c, seed=1, permute_seed=0, wide_offsets=False):
         """rmat(...) built on the GPU, straight into HBM; the same graph bit for
         bit as ctx.upload(rmat(...)) (testData/graphData/rMatGraph.C:33-79)."""

@@ -201,4 +201,40 @@ struct MisArgs {
   int32_t pad;
 };
 
+// ------------------------------------------------------- spanning forest
+// One run of pbbs_spanning_forest (st.hip).  The control words of the round
+// schedule (common/speculative_for.h:30-100) live in device memory.
+struct StCtl {
+  long long done;          // iterations finished (numberDone)
+  long long nkeep;         // failed iterates carried to the front of this round
+  long long cur;           // currentRoundSize
+  long long size;          // this round's iterates
+  long long scatter_size;  // the round just scheduled, for its pack kernel (0: none)
+  long long rounds;        // rounds scheduled so far
+  long long count;         // forest edges, after the output scan
+  int error;               // 1 an edge end outside [0, n), 2 the schedule gave up
+  int finished;
+};
+
+constexpr int kStMaxGroups = 4096;   // workgroups of a round launch at most
+constexpr int kStScanPer = 4;        // counts each thread of the one-workgroup scan sums
+
+struct StArgs {
+  const int32_t* eu;       // [m] edge ends
+  const int32_t* ev;
+  int32_t* parents;        // [n] union-find, -1 at roots
+  uint32_t* R;             // [n] reservations, UINT_MAX when free
+  uint32_t* I;             // [max_round] this round's iterates
+  int32_t* su;             // [max_round] their roots, su < sv
+  int32_t* sv;
+  uint8_t* keep;           // [max_round] reserved and not yet committed
+  uint32_t* hold[2];       // [max_round] failed iterates, by round parity
+  unsigned* blk;           // [kStMaxGroups] per-workgroup counts, then offsets
+  StCtl* ctl;
+  int32_t* host_status;    // pinned: [0] done, [1] rounds
+  long long n, m;
+  long long max_round;     // m / granularity + 1
+  long long max_tries;     // 100 + 200 * granularity
+};
+
 }  // namespace pbbs

@@ -62,6 +62,9 @@ hipError_t launch_mis_round(const MisArgs<OffT>& a, int step, int grid, hipStrea
 template <typename OffT>
 hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
                              unsigned long long* members, int grid, hipStream_t stream);
+hipError_t launch_st_init(const StArgs& a, int grid, hipStream_t stream);
+hipError_t launch_st_round(const StArgs& a, int grid, hipStream_t stream);
+hipError_t launch_st_output(const StArgs& a, int32_t* forest, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_pull_rows(const OffT* off, const int32_t* rows, long long nrows, long long m,
                             long long waves, int32_t* out, hipStream_t stream);
@@ -220,6 +223,18 @@ struct pbbs_ctx {
   unsigned long long* s_ctl = nullptr;   // kMisCtlWords
   unsigned long long* s_part = nullptr;  // kMaxGrid
   uint32_t* s_wg = nullptr;              // kMaxGrid
+  // spanning forest: union-find and reservations per vertex, the round's
+  // iterates and their roots, the two carry lists, per-workgroup counts, the
+  // schedule's control words, and the edges when the caller's are in host
+  // memory
+  int64_t cap_tn = 0, cap_tr = 0, cap_te = 0;
+  int32_t* t_parents = nullptr;
+  uint32_t* t_R = nullptr;
+  unsigned char* t_round = nullptr;      // I | su | sv | hold0 | hold1 | keep
+  unsigned* t_blk = nullptr;
+  StCtl* t_ctl = nullptr;
+  int32_t* t_edges = nullptr;            // eu | ev, cap_te each
+  int32_t* t_out = nullptr;              // cap_tn forest entries for a host copy
 };
 
 struct pbbs_graph {
@@ -295,6 +310,47 @@ constexpr int kMisCtlMembers = kMisCtlFinished + 8;
 constexpr int kMisCtlWords = kMisCtlMembers + 8;
 
 static int64_t mis_region(int64_t n) { return (n + kMisStripes - 1) / kMisStripes + kMisMaxChunk; }
+
+static void free_st_workspace(pbbs_ctx* c) {
+  (void)hipFree(c->t_parents);
+  (void)hipFree(c->t_R);
+  (void)hipFree(c->t_round);
+  (void)hipFree(c->t_blk);
+  (void)hipFree(c->t_ctl);
+  (void)hipFree(c->t_edges);
+  (void)hipFree(c->t_out);
+  c->t_parents = c->t_out = c->t_edges = nullptr;
+  c->t_R = nullptr;
+  c->t_round = nullptr;
+  c->t_blk = nullptr;
+  c->t_ctl = nullptr;
+  c->cap_tn = c->cap_tr = c->cap_te = 0;
+}
+
+// Grows the spanning-forest workspace to n vertices, r iterates a round and
+// (host edges) e edges; buffers only grow.
+static int ensure_st_workspace(pbbs_ctx* c, int64_t n, int64_t r, int64_t e) {
+  if (n <= c->cap_tn && r <= c->cap_tr && e <= c->cap_te && c->t_ctl) return PBBS_OK;
+  const int64_t cn = std::max(n, c->cap_tn), cr = std::max(r, c->cap_tr),
+                ce = std::max(e, c->cap_te);
+  free_st_workspace(c);
+  if (hipMalloc(&c->t_parents, std::max<int64_t>(cn, 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->t_R, std::max<int64_t>(cn, 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&c->t_out, std::max<int64_t>(cn, 1) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&c->t_round, std::max<int64_t>(cr, 1) * 21) != hipSuccess ||
+      hipMalloc(&c->t_blk, kStMaxGroups * sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&c->t_ctl, sizeof(StCtl)) != hipSuccess ||
+      (ce > 0 && hipMalloc(&c->t_edges, 2 * ce * sizeof(int32_t)) != hipSuccess)) {
+    (void)hipGetLastError();
+    free_st_workspace(c);
+    return fail(PBBS_ERR_NOMEM, "forest workspace allocation (n=%lld, round=%lld) failed",
+                (long long)n, (long long)r);
+  }
+  c->cap_tn = cn;
+  c->cap_tr = cr;
+  c->cap_te = ce;
+  return PBBS_OK;
+}
 
 static void free_mis_workspace(pbbs_ctx* c) {
   (void)hipFree(c->s_flags);
@@ -437,6 +493,7 @@ int pbbs_ctx_destroy(pbbs_ctx* c) {
   for (auto& kv : c->batch_graphs) (void)hipGraphExecDestroy(kv.second);
   free_workspace(c);
   free_mis_workspace(c);
+  free_st_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_q);
@@ -1280,6 +1337,127 @@ extern "C" int pbbs_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* flags, uint32_t out
   return PBBS_OK;
 }
 
+// ----------------------------------------------------------- spanning forest
+// The round schedule's constants for granularity 100 (spanningForest/
+// incrementalST/ST.C:69; common/speculative_for.h:33-38).
+constexpr int64_t kStGranularity = 100;
+
+extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int32_t* u,
+                                    const int32_t* v, uint32_t flags, int32_t* forest,
+                                    int64_t* count, pbbs_st_stats* stats) {
+  clear_error();
+  if (!c || !count) return fail(PBBS_ERR_INVALID, "NULL context or count");
+  if (flags & ~(PBBS_OUT_DEVICE | PBBS_EDGES_DEVICE))
+    return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", flags);
+  if (n <= 0 || n > INT32_MAX) return fail(PBBS_ERR_LIMIT, "n=%lld outside [1, 2^31)", (long long)n);
+  if (m < 0 || m >= (int64_t)UINT32_MAX)
+    return fail(PBBS_ERR_LIMIT, "m=%lld outside [0, 2^32 - 1)", (long long)m);
+  if (m > 0 && (!u || !v)) return fail(PBBS_ERR_INVALID, "NULL edge array");
+  const int64_t max_round = m / kStGranularity + 1;
+  if (m > 0 && max_round / 4 == 0)
+    return fail(PBBS_ERR_LIMIT,
+                "m=%lld < %lld: the reference's round schedule starts at size 0 and gives up "
+                "(common/speculative_for.h:36-40)",
+                (long long)m, (long long)(3 * kStGranularity));
+  HIP_TRY(hipSetDevice(c->device));
+  const bool dev_edges = (flags & PBBS_EDGES_DEVICE) != 0;
+  int rc = ensure_st_workspace(c, n, max_round, dev_edges ? 0 : m);
+  if (rc != PBBS_OK) return rc;
+  StArgs a{};
+  a.n = n;
+  a.m = m;
+  a.max_round = max_round;
+  a.max_tries = 100 + 200 * kStGranularity;
+  a.parents = c->t_parents;
+  a.R = c->t_R;
+  a.I = reinterpret_cast<uint32_t*>(c->t_round);
+  a.su = reinterpret_cast<int32_t*>(c->t_round + 4 * c->cap_tr);
+  a.sv = reinterpret_cast<int32_t*>(c->t_round + 8 * c->cap_tr);
+  a.hold[0] = reinterpret_cast<uint32_t*>(c->t_round + 12 * c->cap_tr);
+  a.hold[1] = reinterpret_cast<uint32_t*>(c->t_round + 16 * c->cap_tr);
+  a.keep = c->t_round + 20 * c->cap_tr;
+  a.blk = c->t_blk;
+  a.ctl = c->t_ctl;
+  a.host_status = c->host_status;
+  double copy_ms = 0.0;
+  if (dev_edges) {
+    a.eu = u;
+    a.ev = v;
+  } else {
+    a.eu = c->t_edges;
+    a.ev = c->t_edges + c->cap_te;
+    HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
+    if (m > 0) {
+      HIP_TRY(hipMemcpyAsync(c->t_edges, u, m * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(c->t_edges + c->cap_te, v, m * sizeof(int32_t),
+                             hipMemcpyHostToDevice, c->stream));
+    }
+    HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
+  }
+  // round launches: workgroups for the largest round, 4 per CU at least
+  const int64_t tiles = (max_round + kBlock - 1) / kBlock;
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(
+                       std::min<int64_t>(tiles, kStMaxGroups), std::max(4 * c->num_cus, 1)));
+  const int init_grid = (int)std::min<int64_t>(std::max<int64_t>((n + kBlock - 1) / kBlock, 1),
+                                               8 * c->num_cus);
+  volatile int32_t* hs = c->host_status;
+  HIP_TRY(hipEventRecord(c->ev_start, c->stream));
+  HIP_TRY(launch_st_init(a, init_grid, c->stream));
+  // at least ceil(m / max_round) rounds run (granularity of them at full
+  // size), so the first chunk launches that many; then chunks of 8
+  const int64_t limit = a.max_tries + 2;
+  int64_t launched = 0;
+  int K = (int)std::min<int64_t>(std::max<int64_t>((m + max_round - 1) / max_round, 8), limit);
+  for (;;) {
+    const int k = (int)std::min<int64_t>(K, limit - launched);
+    for (int i = 0; i < k; ++i) HIP_TRY(launch_st_round(a, grid, c->stream));
+    launched += k;
+    HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_end));
+    if (hs[0]) break;
+    if (launched >= limit)
+      return fail(PBBS_ERR_STATE, "forest rounds did not finish in %lld launches",
+                  (long long)launched);
+    K = 8;
+  }
+  int32_t* d_out = (flags & PBBS_OUT_DEVICE) && forest ? forest : c->t_out;
+  HIP_TRY(launch_st_output(a, d_out, grid, c->stream));
+  StCtl hc{};
+  HIP_TRY(hipMemcpyAsync(&hc, c->t_ctl, sizeof hc, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipEventRecord(c->ev_end, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev_end));
+  if (hc.error == 1) return fail(PBBS_ERR_INVALID, "an edge names a vertex outside [0, %lld)",
+                                 (long long)n);
+  if (hc.error == 2) return fail(PBBS_ERR_STATE, "the round schedule gave up after %lld rounds",
+                                 (long long)hc.rounds);
+  float dev_ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_end));
+  if (!dev_edges) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1));
+    copy_ms = ms;
+  }
+  *count = hc.count;
+  if (!(flags & PBBS_OUT_DEVICE) && forest && hc.count > 0) {
+    HIP_TRY(hipEventRecord(c->ev_c0, c->stream));
+    HIP_TRY(hipMemcpyAsync(forest, d_out, hc.count * sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev_c1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev_c0, c->ev_c1));
+    copy_ms += ms;
+  }
+  if (stats) {
+    stats->rounds = (int32_t)hc.rounds;
+    stats->pad = 0;
+    stats->edges = hc.count;
+    stats->device_ms = dev_ms;
+    stats->copy_ms = copy_ms;
+  }
+  return PBBS_OK;
+}
+
 // ------------------------------------------------------------ level profile
 template <typename OffT>
 static int run_profile(pbbs_ctx* c, pbbs_graph* g, int32_t s, pbbs_level_profile* out,
@@ -1746,6 +1924,7 @@ int64_t ctx_workspace_bytes(const pbbs_ctx* c) {
          2 * (int64_t)kSubQueues * c->cap_bn * 8;
   if (c->b_parent) b += 2 * c->cap_bout * 4;
   if (c->b_parent_alt) b += c->cap_bout * 4;
+  if (c->t_ctl) b += 12 * c->cap_tn + 21 * c->cap_tr + 8 * c->cap_te + kStMaxGroups * 4;
   if (c->s_flags)
     b += 9 * c->cap_sn + 8 * kMisStripes * mis_region(c->cap_sn) + kMisCtlWords * 8 +
          kMaxGrid * 12;
@@ -1779,6 +1958,7 @@ void ctx_trim(pbbs_ctx* c) {
   (void)hipStreamSynchronize(c->side);
   free_workspace(c);
   free_mis_workspace(c);
+  free_st_workspace(c);
   (void)hipFree(c->b_masks);
   (void)hipFree(c->b_dlev);
   (void)hipFree(c->b_q);

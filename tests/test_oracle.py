@@ -195,3 +195,84 @@ def test_check_mis_rejects_planted_mistakes():
     bad = f.copy()
     bad[v] = 2
     assert oracle.check_mis(g.offsets, g.adj, bad) >= 0
+
+
+# ---- spanning forest (incrementalST under speculative_for's schedule) ------
+
+def _st_edges(seed, n, m, loops=True):
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, n, m).astype(np.int32)
+    v = rng.integers(0, n, m).astype(np.int32)
+    if not loops:
+        v = np.where(u == v, (v + 1) % n, v).astype(np.int32)
+    return u, v
+
+
+def test_st_schedule_decides_the_forest():
+    """Hand-derived: with m = 1200 the first round holds edges 0-2 (size
+    (1200 / 100 + 1) / 4 = 3).  Edges 0 = (1,2) and 1 = (0,2) both reserve
+    R[2]; edge 0 wins and links 2 under 1.  Edge 2 = (0,1) reserves R[1] and
+    links 1 under 0 in the same round, so edge 1 later finds one tree and is
+    dropped.  Output R in vertex order: R[1] = 2, R[2] = 0.  A serial pass
+    keeps edges 0 and 1 instead."""
+    u = np.r_[[1, 0, 0], np.full(1197, 3)].astype(np.int32)
+    v = np.r_[[2, 2, 1], np.full(1197, 3)].astype(np.int32)
+    f, rounds = oracle.st(4, u, v)
+    assert f.tolist() == [2, 0]
+    assert oracle.serial_st(4, u, v).tolist() == [0, 1]
+    # a round holds at most 1200 / 100 + 1 = 13 iterates
+    assert oracle.check_st(4, u, v, f) == 0 and rounds >= -(-1200 // 13)
+
+
+def test_st_closed_forms():
+    # star with the hub last: edge 0 takes R[hub]; every other edge i then
+    # joins leaf i to the tree of 0 through R[i]
+    n = 2001
+    f, _ = oracle.st(n, np.arange(n - 1), np.full(n - 1, n - 1))
+    assert f.tolist() == list(range(1, n - 1)) + [0]
+    # a path listed in order: edge i = (i, i+1) always holds R[i+1] alone
+    f, _ = oracle.st(n, np.arange(n - 1), np.arange(1, n))
+    assert f.tolist() == list(range(n - 1))
+    # no edges at all; and m < 300, where the reference's schedule gives up
+    f, _ = oracle.st(5, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert f.tolist() == []
+    f, rounds = oracle.st(5, np.zeros(299, np.int32), np.ones(299, np.int32))
+    assert f is None and rounds > 100 + 200 * 100
+    f, _ = oracle.st(5, np.zeros(300, np.int32), np.ones(300, np.int32))
+    assert f.tolist() == [0]
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_st_is_a_spanning_forest(seed):
+    """The schedule's forest spans what the serial forest spans (same size,
+    n - components) and has no cycle; it is not always the serial one."""
+    n, m = [(50, 400), (300, 2000), (2000, 5000), (5000, 40000), (100, 3000), (4000, 4000)][seed]
+    u, v = _st_edges(seed, n, m, loops=seed % 2 == 0)
+    f, _ = oracle.st(n, u, v)
+    s = oracle.serial_st(n, u, v)
+    assert f.size == s.size and np.unique(f).size == f.size
+    assert oracle.check_st(n, u, v, f) == 0 and oracle.check_st(n, u, v, s) == 0
+    # components by scipy agree with n - |forest|
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    k, _ = connected_components(coo_matrix((np.ones(m), (u, v)), shape=(n, n)), directed=False)
+    assert f.size == n - k
+
+
+def test_check_st_rejects_planted_mistakes():
+    u, v = _st_edges(7, 500, 3000)
+    f, _ = oracle.st(500, u, v)
+    assert oracle.check_st(500, u, v, f) == 0
+    assert oracle.check_st(500, u, v, f[:-1]) == 1            # one edge short
+    # swap a forest edge for a non-forest edge closing a cycle
+    inf = set(f.tolist())
+    extra = next(i for i in range(3000) if i not in inf and u[i] != v[i])
+    bad = np.r_[f[:-1], extra].astype(np.int32)
+    s = oracle.serial_st(500, u[bad], v[bad])
+    assert oracle.check_st(500, u, v, bad) in (0, 2) and (s.size < bad.size) == (
+        oracle.check_st(500, u, v, bad) == 2)
+    dup = np.r_[f[:-1], f[0]].astype(np.int32)                 # a repeated edge is a cycle
+    assert oracle.check_st(500, u, v, dup) == 2
+    out = f.copy()
+    out[0] = 3000                                              # not an edge id
+    assert oracle.check_st(500, u, v, out) == 3
