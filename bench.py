@@ -828,6 +828,65 @@ def run_mis(args):
     return out, exact
 
 
+ST_GRAPH = ("rMatGraph_E_12_16000000's shape (one of the three inputs of the spanning-forest "
+            "benchmark, spanningForest/bench/testInputs): 2^24 vertices, 12 * 16M RMAT draws "
+            "(a=.55 b=c=.125), relabelled; one (u, v) per undirected edge in a seeded shuffled "
+            "order, as the unordered edge files are written; built in HBM")
+
+
+def run_st(args):
+    """spanningForest (incrementalST/ST.C:35-71 under speculative_for's round
+    schedule) on ST_GRAPH with the edges and the forest in HBM: device time per
+    forest; with a CPU budget, the oracle's serial restatement of the same
+    schedule timed on the same edges and compared bit for bit."""
+    import pbbsbench_amd as pb
+    ctx = pb.Context(0)
+    t0 = time.time()
+    dg = ctx.gen_rmat(24, 12 * 16_000_000, 0.55, 0.125, 0.125, seed=1, permute_seed=2)
+    n = dg.n
+    e = dg.edges_device(shuffle_seed=3)
+    dg.free()
+    gen_s = time.time() - t0
+    try:
+        for _ in range(max(args.warmup, 1)):
+            ctx.spanning_forest_device(n, e.m, e.u, e.v)
+        dev, t0 = [], time.perf_counter()
+        for _ in range(args.steps):
+            cnt, st = ctx.spanning_forest_device(n, e.m, e.u, e.v)
+            dev.append(st["device_ms"])
+        wall_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        u, v = e.download()
+    finally:
+        e.free()
+    ev = float(np.mean(dev))
+    out = {
+        "graph": ST_GRAPH, "n": n, "m": int(u.size), "gen_s": round(gen_s, 3),
+        "event_ms_per_forest": round(ev, 4),
+        "ms_per_step": round(wall_ms, 4),
+        "rounds": int(st["rounds"]), "forest_edges": int(cnt),
+        "medges_per_s": round(u.size / (ev / 1e3) / 1e6, 1),
+    }
+    exact = True
+    if args.cpu_seconds > 0:
+        import oracle  # the checker and the CPU baseline; never the measured product
+        f, _ = ctx.spanning_forest(n, u, v)
+        t1 = time.perf_counter()
+        want, rounds = oracle.st(n, u, v)
+        cpu_s = time.perf_counter() - t1
+        same = bool(np.array_equal(f, want)) and rounds == st["rounds"]
+        exact = same and oracle.check_st(n, u, v, f) == 0
+        out["cpu_baseline"] = {
+            "value": round(cpu_s * 1e3, 1), "unit": "ms per forest", "cores": 1, "kind": "port",
+            "sample": ("one forest on the same edges by the oracle's serial C restatement of "
+                       f"incrementalST and its round schedule; CPU: {cpu_info()}"),
+        }
+        out["parity"] = ("bit-exact forest ids and round count vs oracle, STCheck rule holds"
+                         if exact else "MISMATCH")
+    ctx.close()
+    say(f"st: {out['event_ms_per_forest']} ms per forest (device), {out['rounds']} rounds")
+    return out, exact
+
+
 def line(base, **over):
     """A stand-alone JSON line for the batch / serve modes."""
     out = {"n_gpus": 1, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
@@ -849,19 +908,23 @@ def main():
     ap.add_argument("--td-to-bu-div", type=int, default=None)
     ap.add_argument("--bu-to-td-div", type=int, default=None)
     ap.add_argument("--mode", default="single",
-                    choices=["single", "batch", "serve", "multi", "ingest", "mis"],
-                    help="single: one-source BFS with the batched, serving, multi-graph and "
-                         "independent-set parts nested in its line (default); batch: 64 sources "
+                    choices=["single", "batch", "serve", "multi", "ingest", "mis", "st"],
+                    help="single: one-source BFS with the batched, serving, multi-graph, "
+                         "independent-set and spanning-forest parts nested in its line "
+                         "(default); batch: 64 sources "
                          "in one traversal vs 64 sequential runs on RMAT 2^22 alone; serve: "
                          "Poisson query stream over 8 resident RMAT graphs alone; ingest: graph "
                          "file (text / binary) into HBM; mis: maximal independent set on "
-                         "rMatGraph_JR_12_16000000 alone")
+                         "rMatGraph_JR_12_16000000 alone; st: spanning forest of the "
+                         "rMatGraph_E_12_16000000 shape alone")
     ap.add_argument("--no-batched", action="store_true", help="single: skip the batched part")
     ap.add_argument("--no-serving", action="store_true", help="single: skip the serving part")
     ap.add_argument("--no-any-parent", action="store_true",
                     help="single: skip timing the any-parent (backForwardBFS) contract")
     ap.add_argument("--no-mis", action="store_true",
                     help="single: skip the maximal-independent-set part")
+    ap.add_argument("--no-st", action="store_true",
+                    help="single: skip the spanning-forest part")
     ap.add_argument("--no-multi", action="store_true",
                     help="single: skip the several-graphs-in-one-launch part")
     ap.add_argument("--no-tuning-table", action="store_true",
@@ -921,6 +984,16 @@ def main():
             "parity": res.get("parity", "GPU check only")},
             **({"cpu_baseline": res.pop("cpu_baseline")} if "cpu_baseline" in res else {}))))
         sys.exit(0 if exact else 1)
+    if args.mode == "st":
+        res, exact = run_st(args)
+        print(json.dumps(line({
+            "metric": "spanning forest of the rMatGraph_E_12_16000000 shape: ms per forest",
+            "value": res["event_ms_per_forest"], "unit": "ms", "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+            "higher_is_better": False, "config": res,
+            "parity": res.get("parity", "not checked (no CPU budget)")},
+            **({"cpu_baseline": res.pop("cpu_baseline")} if "cpu_baseline" in res else {}))))
+        sys.exit(0 if exact else 1)
     if args.mode == "serve":
         res, exact = run_serve(args, nested=False)
         print(json.dumps(line({
@@ -944,6 +1017,10 @@ def main():
     if not args.no_mis:
         say("maximal independent set")
         result["mis"], ok = run_mis(args)
+        exact &= ok
+    if not args.no_st:
+        say("spanning forest")
+        result["spanning_forest"], ok = run_st(args)
         exact &= ok
     print(json.dumps(result))
     if not exact:

@@ -460,6 +460,17 @@ int pbbs_csr_permute(const pbbs_csr* g, uint64_t seed, pbbs_csr* out);
 int pbbs_graph_gen_rmat(pbbs_ctx* ctx, int32_t scale, int64_t draws, double a, double b, double c,
                         uint64_t seed, uint64_t permute_seed, uint32_t flags, pbbs_graph** out);
 int pbbs_graph_gen_grid3d(pbbs_ctx* ctx, int32_t d, uint32_t flags, pbbs_graph** out);
+/* The undirected edges of a resident symmetric graph as an edge array in
+ * HBM: one (u[i], v[i]) with u < v per edge, in row order, or in the order of
+ * a seeded random shuffle when shuffle_seed != 0 (the unordered edge files
+ * the spanning-forest benchmark reads are written shuffled:
+ * common/graphIO.h:319-333).  u and v are device buffers with room for cap
+ * entries; *m_out gets the edge count (also when cap is too small, which is
+ * PBBS_ERR_LIMIT; u = v = NULL, cap = 0 asks for the count only).  Replaces
+ * writing an EdgeArray file (edgesFromGraph + randomShuffle) and reading it
+ * back (common/graphIO.h:147-166). */
+int pbbs_graph_edges(pbbs_ctx* ctx, const pbbs_graph* g, uint64_t shuffle_seed, int32_t* u,
+                     int32_t* v, int64_t cap, int64_t* m_out);
 /* Copies a resident graph back into a host CSR (int64 offsets; free it with
  * pbbs_csr_free).  Replaces writeGraphToFile's source of data
  * (common/graphIO.h:78-105). */

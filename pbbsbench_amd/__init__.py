@@ -5,6 +5,7 @@ from ._lib import PbbsError, LIB_PATH  # noqa: F401
 from .core import (  # noqa: F401
     BFS,
     Context,
+    DeviceEdges,
     DeviceGraph,
     HostGraph,
     Server,
@@ -22,7 +23,7 @@ from .core import (  # noqa: F401
 )
 
 __all__ = [
-    "BFS", "Context", "DeviceGraph", "HostGraph", "PbbsError", "Server", "from_edges", "grid3d",
+    "BFS", "Context", "DeviceEdges", "DeviceGraph", "HostGraph", "PbbsError", "Server", "from_edges", "grid3d",
     "permute", "randlocal", "read_graph_binary", "read_graph_from_file", "read_int_seq_from_file", "rmat",
     "write_graph_binary", "write_graph_to_file", "write_sequence_to_file",
 ]

@@ -212,6 +212,8 @@ SIGNATURES = [
       C.c_uint32, C.POINTER(_P)]),
     ("pbbs_graph_gen_grid3d", C.c_int, [_P, C.c_int32, C.c_uint32, C.POINTER(_P)]),
     ("pbbs_graph_download", C.c_int, [_P, C.POINTER(Csr)]),
+    ("pbbs_graph_edges", C.c_int,
+     [_P, _P, C.c_uint64, _P, _P, C.c_int64, C.POINTER(C.c_int64)]),
     ("pbbs_ctx_outputs", C.c_int, [_P, C.c_int, C.POINTER(_P), C.POINTER(_P)]),
     ("pbbs_check_bfs", C.c_int,
      [_P, _P, C.c_int32, _P, _P, C.c_uint32, C.POINTER(CheckResult)]),

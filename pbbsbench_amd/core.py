@@ -18,6 +18,7 @@ writeSequenceToFile (common/sequenceIO.h:189-193).  Here:
 Every call goes through libpbbs_bfs.so; nothing here computes a BFS on the CPU.
 """
 import ctypes as C
+import os
 import threading
 import time
 
@@ -383,6 +384,22 @@ class DeviceGraph:
                                    _lib.PBBS_CHECK_DEVICE_INPUT, C.byref(r)), "check_bfs")
         return r.as_dict()
 
+    def edges_device(self, shuffle_seed=0):
+        """The undirected edges (u < v) of this symmetric graph as an edge array
+        in HBM, in row order or shuffled by a seeded permutation.  Returns a
+        DeviceEdges (free it when done)."""
+        m = C.c_int64()
+        check(lib().pbbs_graph_edges(self.ctx._h, self._h, 0, None, None, 0, C.byref(m)),
+              "graph_edges")
+        e = DeviceEdges(self.n, m.value)
+        try:
+            check(lib().pbbs_graph_edges(self.ctx._h, self._h, int(shuffle_seed), e.u, e.v,
+                                         e.m, C.byref(m)), "graph_edges")
+        except Exception:
+            e.free()
+            raise
+        return e
+
     def download(self):
         """The resident CSR as a HostGraph (int64 offsets)."""
         c = _lib.Csr()
@@ -491,6 +508,61 @@ class DeviceGraph:
         check(lib().pbbs_bfs(self.ctx._h, self._h, int(source), parent_ptr, depth_ptr,
                              flags, C.byref(st)), "BFS")
         return st.as_dict()
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime library, for device buffers the caller owns."""
+    global _hip
+    if _hip is None:
+        root = os.environ.get("ROCM_PATH", "/opt/rocm")
+        h = C.CDLL(os.path.join(root, "lib", "libamdhip64.so"))
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _hip = h
+    return _hip
+
+
+class DeviceEdges:
+    """An edge array of m (u, v) int32 pairs in HBM on n vertices (the
+    reference's edgeArray, resident).  u and v are device addresses."""
+
+    def __init__(self, n, m):
+        self.n, self.m = int(n), int(m)
+        self._u, self._v = C.c_void_p(), C.c_void_p()
+        h = hip_runtime()
+        if h.hipMalloc(C.byref(self._u), max(self.m, 1) * 4) != 0 or \
+                h.hipMalloc(C.byref(self._v), max(self.m, 1) * 4) != 0:
+            self.free()
+            raise MemoryError(f"device edge array of {m} entries")
+
+    @property
+    def u(self):
+        return self._u.value
+
+    @property
+    def v(self):
+        return self._v.value
+
+    def download(self):
+        """(u, v) as host int32 arrays."""
+        u = np.empty(self.m, dtype=np.int32)
+        v = np.empty(self.m, dtype=np.int32)
+        h = hip_runtime()
+        if self.m:
+            assert h.hipMemcpy(u.ctypes.data, self._u, self.m * 4, 2) == 0
+            assert h.hipMemcpy(v.ctypes.data, self._v, self.m * 4, 2) == 0
+        return u, v
+
+    def free(self):
+        h = hip_runtime()
+        for p in (self._u, self._v):
+            if p.value:
+                h.hipFree(p)
+                p.value = None
 
 
 class Server:

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 
 #include "../../include/pbbs_bfs.h"
@@ -29,6 +30,13 @@ namespace pbbs {
 hipStream_t ctx_stream(pbbs_ctx* c);
 int ctx_device(const pbbs_ctx* c);
 int ctx_num_cus(const pbbs_ctx* c);
+const void* graph_dev_off(const pbbs_graph* g);
+const int32_t* graph_dev_adj(const pbbs_graph* g);
+bool graph_off64(const pbbs_graph* g);
+bool graph_symmetric(const pbbs_graph* g);
+int64_t graph_n(const pbbs_graph* g);
+int64_t graph_m(const pbbs_graph* g);
+int graph_device(const pbbs_graph* g);
 }  // namespace pbbs
 
 namespace {
@@ -205,9 +213,125 @@ int finish_graph(pbbs_ctx* c, int64_t n, int vb, unsigned long long* keys, int64
                            out);
 }
 
+// Edge arrays from a resident graph: per row, the entries above the row id.
+template <typename OffT>
+__global__ void upper_count_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ adj,
+                                   long long n, unsigned long long* __restrict__ cnt) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    unsigned long long c = 0;
+    for (OffT e = off[v]; e < off[v + 1]; ++e) c += adj[e] > v;
+    cnt[v] = c;
+  }
+}
+
+// Row-order pairs packed (u << 32 | v), u < v, at each row's prefix position.
+template <typename OffT>
+__global__ void upper_write_kernel(const OffT* __restrict__ off, const int32_t* __restrict__ adj,
+                                   long long n, const unsigned long long* __restrict__ pre,
+                                   unsigned long long* __restrict__ pairs) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+    unsigned long long w = pre[v];
+    for (OffT e = off[v]; e < off[v + 1]; ++e)
+      if (adj[e] > v) pairs[w++] = ((unsigned long long)v << 32) | (uint32_t)adj[e];
+  }
+}
+
+__global__ void shuffle_keys_kernel(long long m, uint64_t hseed, unsigned long long* __restrict__ k) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride)
+    k[i] = mix64(hseed ^ ((uint64_t)i * 0xD1B54A32D192ED03ull));
+}
+
+__global__ void unpack_pairs_kernel(long long m, const unsigned long long* __restrict__ pairs,
+                                    int32_t* __restrict__ u, int32_t* __restrict__ v) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    const unsigned long long p = pairs[i];
+    u[i] = (int32_t)(p >> 32);
+    v[i] = (int32_t)(uint32_t)p;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+int pbbs_graph_edges(pbbs_ctx* c, const pbbs_graph* g, uint64_t shuffle_seed, int32_t* u,
+                     int32_t* v, int64_t cap, int64_t* m_out) {
+  pbbs::clear_error();
+  if (!c || !g || !m_out) return fail(PBBS_ERR_INVALID, "NULL argument");
+  if (!pbbs::graph_symmetric(g)) return fail(PBBS_ERR_INVALID, "the graph is not symmetric");
+  if (pbbs::graph_device(g) != pbbs::ctx_device(c))
+    return fail(PBBS_ERR_INVALID, "graph lives on another device");
+  GEN_TRY(hipSetDevice(pbbs::ctx_device(c)));
+  hipStream_t st = pbbs::ctx_stream(c);
+  const int grid = std::max(1, pbbs::ctx_num_cus(c) * 8);
+  const int64_t n = pbbs::graph_n(g);
+  DevBufs bufs;
+  unsigned long long *cnt = nullptr, *pre = nullptr;
+  GEN_TRY(bufs.alloc(&cnt, (n + 1) * sizeof(unsigned long long)));
+  GEN_TRY(bufs.alloc(&pre, (n + 1) * sizeof(unsigned long long)));
+  const bool w = pbbs::graph_off64(g);
+  const void* off = pbbs::graph_dev_off(g);
+  const int32_t* adj = pbbs::graph_dev_adj(g);
+  if (w)
+    hipLaunchKernelGGL(upper_count_kernel<int64_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const int64_t*>(off), adj, (long long)n, cnt);
+  else
+    hipLaunchKernelGGL(upper_count_kernel<uint32_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const uint32_t*>(off), adj, (long long)n, cnt);
+  GEN_TRY(hipGetLastError());
+  GEN_TRY(hipMemsetAsync(cnt + n, 0, sizeof(unsigned long long), st));
+  size_t b1 = 0;
+  GEN_TRY(rocprim::exclusive_scan(nullptr, b1, cnt, pre, 0ull, (size_t)(n + 1),
+                                  rocprim::plus<unsigned long long>(), st));
+  void* tmp = nullptr;
+  GEN_TRY(bufs.alloc(&tmp, b1));
+  GEN_TRY(rocprim::exclusive_scan(tmp, b1, cnt, pre, 0ull, (size_t)(n + 1),
+                                  rocprim::plus<unsigned long long>(), st));
+  unsigned long long m = 0;
+  GEN_TRY(hipMemcpyAsync(&m, pre + n, sizeof m, hipMemcpyDeviceToHost, st));
+  GEN_TRY(hipStreamSynchronize(st));
+  *m_out = (int64_t)m;
+  if (!u && !v && cap == 0) return PBBS_OK;   // the count only
+  if ((int64_t)m > cap) return fail(PBBS_ERR_LIMIT, "%lld edges, room for %lld", (long long)m,
+                                    (long long)cap);
+  if (m == 0) return PBBS_OK;
+  if (!u || !v) return fail(PBBS_ERR_INVALID, "NULL edge buffer");
+  unsigned long long* pairs = nullptr;
+  GEN_TRY(bufs.alloc(&pairs, m * sizeof(unsigned long long)));
+  if (w)
+    hipLaunchKernelGGL(upper_write_kernel<int64_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const int64_t*>(off), adj, (long long)n, pre, pairs);
+  else
+    hipLaunchKernelGGL(upper_write_kernel<uint32_t>, dim3(grid), dim3(256), 0, st,
+                       static_cast<const uint32_t*>(off), adj, (long long)n, pre, pairs);
+  GEN_TRY(hipGetLastError());
+  const unsigned long long* src = pairs;
+  if (shuffle_seed) {
+    unsigned long long *keys = nullptr, *skeys = nullptr, *spairs = nullptr;
+    GEN_TRY(bufs.alloc(&keys, m * sizeof(unsigned long long)));
+    GEN_TRY(bufs.alloc(&skeys, m * sizeof(unsigned long long)));
+    GEN_TRY(bufs.alloc(&spairs, m * sizeof(unsigned long long)));
+    hipLaunchKernelGGL(shuffle_keys_kernel, dim3(grid), dim3(256), 0, st, (long long)m,
+                       mix64_host(shuffle_seed), keys);
+    GEN_TRY(hipGetLastError());
+    size_t b2 = 0;
+    GEN_TRY(rocprim::radix_sort_pairs(nullptr, b2, keys, skeys, pairs, spairs, (size_t)m, 0, 64,
+                                      st));
+    void* tmp2 = nullptr;
+    GEN_TRY(bufs.alloc(&tmp2, b2));
+    GEN_TRY(rocprim::radix_sort_pairs(tmp2, b2, keys, skeys, pairs, spairs, (size_t)m, 0, 64,
+                                      st));
+    src = spairs;
+  }
+  hipLaunchKernelGGL(unpack_pairs_kernel, dim3(grid), dim3(256), 0, st, (long long)m, src, u, v);
+  GEN_TRY(hipGetLastError());
+  GEN_TRY(hipStreamSynchronize(st));
+  return PBBS_OK;
+}
 
 int pbbs_graph_gen_rmat(pbbs_ctx* c, int32_t scale, int64_t draws, double a, double b, double cq,
                         uint64_t seed, uint64_t permute_seed, uint32_t flags, pbbs_graph** out) {

@@ -21,7 +21,8 @@ def test_bench_refuses_more_than_one_gpu():
 def test_bench_help_names_every_mode():
     r = run(os.path.join(ROOT, "bench.py"), "--help")
     assert r.returncode == 0
-    for word in ("single", "batch", "serve", "ingest", "mis", "--no-mis", "--spread", "--workload"):
+    for word in ("single", "batch", "serve", "ingest", "mis", "st", "--no-mis", "--no-st",
+                 "--spread", "--workload"):
         assert word in r.stdout
 
 

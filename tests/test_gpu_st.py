@@ -150,3 +150,37 @@ def test_st_rmat22_edges():
         del g
         f, st = run_st(c, 2 ** 22, u, v)
         say(f"rounds={st['rounds']} edges={st['edges']} ms={st['device_ms']:.2f}")
+
+
+def test_graph_edges_on_gpu(ctx):
+    """pbbs_graph_edges: row order equals the CSR's upper entries; a seeded
+    shuffle is a permutation of them, the same for the same seed; the forest
+    of the device edges equals the oracle's on the downloaded ones."""
+    g = pb.rmat(12, 12 * 2 ** 12, 0.55, 0.125, 0.125, seed=4, permute_seed=5)
+    dg = ctx.upload(g)
+    off = np.asarray(g.offsets)
+    src = np.repeat(np.arange(g.n, dtype=np.int32), np.diff(off))
+    keep = src < g.adj
+    want_u, want_v = src[keep], g.adj[keep]
+    es = []
+    try:
+        say("graph edges: row order and two shuffles")
+        for seed in (0, 9, 9, 10):
+            es.append(dg.edges_device(seed))
+        u0, v0 = es[0].download()
+        assert np.array_equal(u0, want_u) and np.array_equal(v0, want_v)
+        u1, v1 = es[1].download()
+        u2, v2 = es[2].download()
+        u3, v3 = es[3].download()
+        assert np.array_equal(u1, u2) and np.array_equal(v1, v2)
+        assert not np.array_equal(u1, u0) and not np.array_equal(u1, u3)
+        k = lambda a, b: np.sort(a.astype(np.int64) << 32 | b.astype(np.int64))
+        assert np.array_equal(k(u1, v1), k(u0, v0)) and np.array_equal(k(u3, v3), k(u0, v0))
+        cnt, st = ctx.spanning_forest_device(g.n, es[1].m, es[1].u, es[1].v)
+        f, _ = ctx.spanning_forest(g.n, u1, v1)
+        want, rounds = oracle.st(g.n, u1, v1)
+        assert cnt == f.size and np.array_equal(f, want) and st["rounds"] == rounds
+    finally:
+        for e in es:
+            e.free()
+        dg.free()
