@@ -1394,10 +1394,11 @@ extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int
     }
     HIP_TRY(hipEventRecord(c->ev_c1, c->stream));
   }
-  // round launches: workgroups for the largest round, 4 per CU at least
+  // round launches: one thread per iterate of the largest round, up to
+  // kStMaxGroups workgroups (the finds are chains of dependent loads, so
+  // more of them in flight is what hides their latency)
   const int64_t tiles = (max_round + kBlock - 1) / kBlock;
-  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(
-                       std::min<int64_t>(tiles, kStMaxGroups), std::max(4 * c->num_cus, 1)));
+  const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, kStMaxGroups));
   const int init_grid = (int)std::min<int64_t>(std::max<int64_t>((n + kBlock - 1) / kBlock, 1),
                                                8 * c->num_cus);
   volatile int32_t* hs = c->host_status;
@@ -1426,10 +1427,10 @@ extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int
   HIP_TRY(hipMemcpyAsync(&hc, c->t_ctl, sizeof hc, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipEventRecord(c->ev_end, c->stream));
   HIP_TRY(hipEventSynchronize(c->ev_end));
-  if (hc.error == 1) return fail(PBBS_ERR_INVALID, "an edge names a vertex outside [0, %lld)",
-                                 (long long)n);
-  if (hc.error == 2) return fail(PBBS_ERR_STATE, "the round schedule gave up after %lld rounds",
-                                 (long long)hc.rounds);
+  if (hc.error & 1) return fail(PBBS_ERR_INVALID, "an edge names a vertex outside [0, %lld)",
+                                (long long)n);
+  if (hc.error & 2) return fail(PBBS_ERR_STATE, "the round schedule gave up after %lld rounds",
+                                (long long)hc.rounds);
   float dev_ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_end));
   if (!dev_edges) {

@@ -84,23 +84,6 @@ __global__ __launch_bounds__(kBlock) void st_init_kernel(StArgs a) {
   }
 }
 
-// Every edge end must name a vertex: a bad one ends the run before any round
-// reads it (the round kernels return at once once `finished` is set).
-__global__ __launch_bounds__(kBlock) void st_validate_kernel(StArgs a) {
-  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nth = (long long)gridDim.x * blockDim.x;
-  bool bad = false;
-  for (long long i = tid; i < a.m; i += nth) {
-    const int32_t u = a.eu[i], v = a.ev[i];
-    bad |= u < 0 || (long long)u >= a.n || v < 0 || (long long)v >= a.n;
-  }
-  if (__ballot(bad) && __lane_id() == 0) {
-    atomicOr(&a.ctl->error, 1);
-    atomicOr(&a.ctl->finished, 1);
-    __hip_atomic_store(&a.host_status[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
-}
-
 __global__ __launch_bounds__(kBlock) void st_reserve_kernel(StArgs a) {
   const StCtl& c = *a.ctl;
   if (c.finished) return;
@@ -110,14 +93,22 @@ __global__ __launch_bounds__(kBlock) void st_reserve_kernel(StArgs a) {
   const long long nth = (long long)gridDim.x * blockDim.x;
   for (long long i = tid; i < size; i += nth) {
     const uint32_t it = i < nk ? hold[i] : (uint32_t)(done + i);
-    int32_t u = st_find(a.parents, a.eu[it]);
-    int32_t v = st_find(a.parents, a.ev[it]);
+    const int32_t eu = a.eu[it], ev = a.ev[it];
+    a.I[i] = it;
+    if ((uint32_t)eu >= (uint32_t)a.n || (uint32_t)ev >= (uint32_t)a.n) {
+      // an end outside [0, n): the run reports it and its forest is not used;
+      // the edge takes no part (it is never read as a vertex)
+      a.keep[i] = 0;
+      atomicOr(&a.ctl->error, 1);
+      continue;
+    }
+    int32_t u = st_find(a.parents, eu);
+    int32_t v = st_find(a.parents, ev);
     if (u > v) {
       const int32_t t = u;
       u = v;
       v = t;
     }
-    a.I[i] = it;
     a.su[i] = u;
     a.sv[i] = v;
     a.keep[i] = u != v;
@@ -209,7 +200,7 @@ __global__ __launch_bounds__(1024) void st_schedule_kernel(StArgs a, int groups)
     c.size = c.cur < a.m - c.done ? c.cur : a.m - c.done;
     int fin = c.done >= a.m;
     if (!fin && c.rounds > a.max_tries) {   // speculative_for gives up (size 0 never moves)
-      c.error = 2;
+      c.error |= 2;
       fin = 1;
     }
     c.finished = fin;
@@ -334,9 +325,6 @@ __global__ __launch_bounds__(kBlock) void st_out_write_kernel(StArgs a, int32_t*
 
 hipError_t launch_st_init(const StArgs& a, int grid, hipStream_t stream) {
   hipLaunchKernelGGL(st_init_kernel, dim3(grid), dim3(kBlock), 0, stream, a);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(st_validate_kernel, dim3(grid), dim3(kBlock), 0, stream, a);
   return hipGetLastError();
 }
 
