@@ -504,6 +504,12 @@ int pbbs_graph_read(pbbs_ctx* ctx, const char* path, uint32_t flags, pbbs_graph*
  * common/IO.h:151-166 via common/sequenceIO.h:189-193, reader
  * common/IO.h:217-229).  The reader allocates *out with malloc. */
 int pbbs_write_sequence_int(const char* path, const int32_t* a, int64_t n);
+/* PBBS EdgeArray text format ("EdgeArray", then u v per edge): reader
+ * common/graphIO.h:147-166 (n = the largest id + 1; an odd last token is
+ * ignored, as there; ids must lie in [0, 2^31 - 1)), writer :136-138.  The
+ * reader's arrays are freed with pbbs_free. */
+int pbbs_read_edge_array(const char* path, int64_t* n, int64_t* m, int32_t** u, int32_t** v);
+int pbbs_write_edge_array(const char* path, int64_t m, const int32_t* u, const int32_t* v);
 int pbbs_read_sequence_int(const char* path, int32_t** out, int64_t* n);
 void pbbs_free(void* p);
 

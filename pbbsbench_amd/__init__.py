@@ -13,17 +13,21 @@ from .core import (  # noqa: F401
     grid3d,
     permute,
     randlocal,
+    read_edge_array_from_file,
     read_graph_binary,
     read_graph_from_file,
     read_int_seq_from_file,
     rmat,
+    write_edge_array_to_file,
     write_graph_binary,
     write_graph_to_file,
     write_sequence_to_file,
 )
 
 __all__ = [
-    "BFS", "Context", "DeviceEdges", "DeviceGraph", "HostGraph", "PbbsError", "Server", "from_edges", "grid3d",
-    "permute", "randlocal", "read_graph_binary", "read_graph_from_file", "read_int_seq_from_file", "rmat",
-    "write_graph_binary", "write_graph_to_file", "write_sequence_to_file",
+    "BFS", "Context", "DeviceEdges", "DeviceGraph", "HostGraph", "PbbsError", "Server",
+    "from_edges", "grid3d", "permute", "randlocal", "read_edge_array_from_file",
+    "read_graph_binary", "read_graph_from_file", "read_int_seq_from_file", "rmat",
+    "write_edge_array_to_file", "write_graph_binary", "write_graph_to_file",
+    "write_sequence_to_file",
 ]

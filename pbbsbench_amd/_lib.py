@@ -244,6 +244,10 @@ SIGNATURES = [
     ("pbbs_graph_read", C.c_int, [_P, C.c_char_p, C.c_uint32, C.POINTER(_P)]),
     ("pbbs_write_sequence_int", C.c_int, [C.c_char_p, _P, C.c_int64]),
     ("pbbs_read_sequence_int", C.c_int, [C.c_char_p, C.POINTER(_I32P), C.POINTER(C.c_int64)]),
+    ("pbbs_read_edge_array", C.c_int,
+     [C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(_I32P),
+      C.POINTER(_I32P)]),
+    ("pbbs_write_edge_array", C.c_int, [C.c_char_p, C.c_int64, _P, _P]),
     ("pbbs_free", None, [_P]),
 ]
 

@@ -173,6 +173,33 @@ def read_int_seq_from_file(path):
         lib().pbbs_free(C.cast(p, C.c_void_p))
 
 
+def read_edge_array_from_file(path):
+    """EdgeArray reader (common/graphIO.h:147-166): (n, u, v), n = the largest
+    id + 1."""
+    pu, pv = C.POINTER(C.c_int32)(), C.POINTER(C.c_int32)()
+    n, m = C.c_int64(), C.c_int64()
+    check(lib().pbbs_read_edge_array(str(path).encode(), C.byref(n), C.byref(m), C.byref(pu),
+                                     C.byref(pv)), "readEdgeArrayFromFile")
+    try:
+        if m.value == 0:
+            return n.value, np.zeros(0, np.int32), np.zeros(0, np.int32)
+        return (n.value, np.ctypeslib.as_array(pu, shape=(m.value,)).copy(),
+                np.ctypeslib.as_array(pv, shape=(m.value,)).copy())
+    finally:
+        lib().pbbs_free(C.cast(pu, C.c_void_p))
+        lib().pbbs_free(C.cast(pv, C.c_void_p))
+
+
+def write_edge_array_to_file(u, v, path):
+    """EdgeArray writer (common/graphIO.h:136-138)."""
+    eu = np.ascontiguousarray(u, dtype=np.int32)
+    ev = np.ascontiguousarray(v, dtype=np.int32)
+    if eu.shape != ev.shape:
+        raise ValueError("u and v differ in length")
+    check(lib().pbbs_write_edge_array(str(path).encode(), eu.size, _ptr(eu), _ptr(ev)),
+          "writeEdgeArrayToFile")
+
+
 # ------------------------------------------------------------ device objects
 class Context:
     """One GPU: a stream, a BFS workspace and captured level graphs."""

@@ -1,5 +1,6 @@
-// PBBS text formats: AdjacencyGraph (docs/fileFormats/graph.md:16-39) and
-// sequenceInt (docs/fileFormats/sequence.md:10-26).  Tokens are separated by
+// PBBS text formats: AdjacencyGraph (docs/fileFormats/graph.md:16-39),
+// EdgeArray (common/graphIO.h:136-166) and sequenceInt
+// (docs/fileFormats/sequence.md:10-26).  Tokens are separated by
 // any run of space, tab, line feed or carriage return (common/IO.h:41-50).
 // Readers parse in parallel chunks split at token boundaries; writers format
 // in parallel blocks and emit one value per line, as common/IO.h:113-166 does.
@@ -141,9 +142,97 @@ int write_lines(FILE* f, const T* a, int64_t n) {
   return PBBS_OK;
 }
 
+// Formats "u v\n" for each pair in parallel blocks and writes them.
+int write_pairs(FILE* f, const int32_t* u, const int32_t* v, int64_t m) {
+  const int64_t block = 1 << 20;
+  const int64_t nb = (m + block - 1) / block;
+  const int T_ = std::max(1, omp_get_max_threads());
+  std::vector<std::vector<char>> out(T_, std::vector<char>(block * 24));
+  std::vector<int64_t> len(T_);
+  for (int64_t b0 = 0; b0 < nb; b0 += T_) {
+    const int64_t b1 = std::min(nb, b0 + T_);
+#pragma omp parallel for schedule(static, 1) num_threads(T_)
+    for (int64_t b = b0; b < b1; ++b) {
+      char* s = out[b - b0].data();
+      char* p = s;
+      const int64_t lo = b * block, hi = std::min(m, lo + block);
+      for (int64_t i = lo; i < hi; ++i) {
+        p = put_line(p, (long long)u[i]);
+        p[-1] = ' ';
+        p = put_line(p, (long long)v[i]);
+      }
+      len[b - b0] = p - s;
+    }
+    for (int64_t b = b0; b < b1; ++b)
+      if (std::fwrite(out[b - b0].data(), 1, (size_t)len[b - b0], f) != (size_t)len[b - b0])
+        return fail(PBBS_ERR_IO, "write failed");
+  }
+  return PBBS_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int pbbs_write_edge_array(const char* path, int64_t m, const int32_t* u, const int32_t* v) {
+  pbbs::clear_error();
+  if (!path || m < 0 || (m > 0 && (!u || !v))) return fail(PBBS_ERR_INVALID, "bad argument");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(PBBS_ERR_IO, "cannot open %s: %s", path, std::strerror(errno));
+  std::fputs("EdgeArray\n", f);
+  int rc = write_pairs(f, u, v, m);
+  if (std::fclose(f) != 0 && rc == PBBS_OK) rc = fail(PBBS_ERR_IO, "close of %s failed", path);
+  return rc;
+}
+
+int pbbs_read_edge_array(const char* path, int64_t* n_out, int64_t* m_out, int32_t** u_out,
+                         int32_t** v_out) {
+  pbbs::clear_error();
+  if (!path || !n_out || !m_out || !u_out || !v_out) return fail(PBBS_ERR_INVALID, "NULL argument");
+  *u_out = *v_out = nullptr;
+  *n_out = *m_out = 0;
+  std::vector<char> buf;
+  int rc = read_file(path, buf);
+  if (rc != PBBS_OK) return rc;
+  int64_t after = 0;
+  if (header_of(buf, &after) != "EdgeArray")
+    return fail(PBBS_ERR_FORMAT, "%s: missing header EdgeArray", path);
+  const std::vector<int64_t> tok = token_starts(buf, after, (int64_t)buf.size());
+  // (tokens - 1) / 2 pairs with the header counted (common/graphIO.h:154): an
+  // odd last token is ignored, as there
+  const int64_t m = (int64_t)tok.size() / 2;
+  int32_t* u = static_cast<int32_t*>(std::malloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
+  int32_t* v = static_cast<int32_t*>(std::malloc(std::max<int64_t>(m, 1) * sizeof(int32_t)));
+  if (!u || !v) {
+    std::free(u);
+    std::free(v);
+    return fail(PBBS_ERR_NOMEM, "allocation failed");
+  }
+  long long bad = -1, mx = 0;
+#pragma omp parallel for schedule(static) reduction(max : bad, mx)
+  for (int64_t i = 0; i < m; ++i) {
+    long long a = 0, b = 0;
+    if (!parse_long(&buf[tok[2 * i]], &a) || !parse_long(&buf[tok[2 * i + 1]], &b) || a < 0 ||
+        b < 0 || a >= INT32_MAX || b >= INT32_MAX) {
+      bad = std::max(bad, (long long)i);
+    } else {
+      u[i] = (int32_t)a;
+      v[i] = (int32_t)b;
+      mx = std::max(mx, std::max(a, b));
+    }
+  }
+  if (bad >= 0) {
+    std::free(u);
+    std::free(v);
+    return fail(PBBS_ERR_FORMAT, "%s: edge %lld is not two vertex ids in [0, 2^31 - 1)", path,
+                bad);
+  }
+  *u_out = u;
+  *v_out = v;
+  *m_out = m;
+  *n_out = mx + 1;   // the largest id + 1 (common/graphIO.h:159-164)
+  return PBBS_OK;
+}
 
 void pbbs_free(void* p) { std::free(p); }
 

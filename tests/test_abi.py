@@ -79,3 +79,21 @@ def test_driver_reads_graph_and_rejects_bad_source():
     out = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "no_such.adj")],
                          capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "readGraphFromFile" in out.stderr
+
+
+@pytest.mark.parametrize("name", ["MIS", "ST"])
+def test_set_and_forest_drivers_usage_and_bad_files(tmp_path, name):
+    """bin/MIS and bin/ST take the reference drivers' command line
+    (MISTime.C:48-55, STTime.C:45-52) and fail on a missing or malformed
+    input before they touch a device."""
+    exe = os.path.join(ROOT, "pbbsbench_amd", "bin", name)
+    assert os.path.exists(exe)
+    out = subprocess.run([exe, "-h"], capture_output=True, text=True, timeout=30)
+    assert out.stdout.startswith("usage:") and "[-o <outFile>] [-r <rounds>] <inFile>" in out.stdout
+    out = subprocess.run([exe, str(tmp_path / "missing")], capture_output=True, text=True,
+                         timeout=60)
+    assert out.returncode == 1 and "cannot open" in out.stderr
+    bad = tmp_path / "bad"
+    bad.write_text("AdjacencyGraph\n2\n1\n0\n" if name == "ST" else "EdgeArray\n0 1\n")
+    out = subprocess.run([exe, str(bad)], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 1 and "missing header" in out.stderr

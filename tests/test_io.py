@@ -143,3 +143,41 @@ def test_binary_rejects_bad_files(tmp_path):
     (tmp_path / "text.adj").write_text("AdjacencyGraph\n1\n0\n0\n")
     with pytest.raises(pb.PbbsError, match="not a PBBSCSR1"):
         pb.read_graph_binary(tmp_path / "text.adj")
+
+
+def test_edge_array_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    u = rng.integers(0, 1000, 5000).astype(np.int32)
+    v = rng.integers(0, 1000, 5000).astype(np.int32)
+    p = tmp_path / "e"
+    pb.write_edge_array_to_file(u, v, p)
+    lines = open(p).read().split("\n")
+    assert lines[0] == "EdgeArray" and lines[1] == f"{u[0]} {v[0]}"
+    n, u2, v2 = pb.read_edge_array_from_file(p)
+    assert n == max(u.max(), v.max()) + 1
+    assert np.array_equal(u2, u) and np.array_equal(v2, v)
+
+
+@pytest.mark.parametrize("body,n,m", [
+    ("EdgeArray\n", 1, 0),                      # no edges: n = 0 + 1, as the reference's reduce
+    ("EdgeArray 3 4\t5\r\n6", 7, 2),            # any separators
+    ("EdgeArray\n1 2\n3", 3, 1),                # an odd last token is ignored
+])
+def test_edge_array_reader_small_files(tmp_path, body, n, m):
+    p = tmp_path / "e"
+    p.write_text(body)
+    got_n, u, v = pb.read_edge_array_from_file(p)
+    assert (got_n, u.size, v.size) == (n, m, m)
+
+
+@pytest.mark.parametrize("body,match", [
+    ("AdjacencyGraph\n1 2\n", "missing header"),
+    ("EdgeArray\n1 x\n", "edge 0"),
+    ("EdgeArray\n0 1\n-3 2\n", "edge 1"),
+    ("EdgeArray\n0 2147483647\n", "edge 0"),
+])
+def test_edge_array_reader_rejects_malformed(tmp_path, body, match):
+    p = tmp_path / "e"
+    p.write_text(body)
+    with pytest.raises(pb.PbbsError, match=match):
+        pb.read_edge_array_from_file(p)
