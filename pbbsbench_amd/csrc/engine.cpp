@@ -1243,9 +1243,9 @@ extern "C" int pbbs_bfs(pbbs_ctx* c, pbbs_graph* g, int32_t source, int32_t* par
 }
 
 // ------------------------------------------------- maximal independent set
-// Rounds run in chunks of K launches on the context's stream (16, then 8:
-// the relabelled benchmark graphs finish in 13-18 rounds, and a launch after
-// the last round still costs its dispatch); after each chunk the host reads whether a round found the worklist
+// Rounds run in chunks of 8 launches on the context's stream (the
+// relabelled benchmark graphs finish in 6-9 rounds, and a launch after the
+// last round still costs its dispatch); after each chunk the host reads whether a round found the worklist
 // empty.  A round decides at least the lowest undecided vertex (its lower
 // neighbours were all decided before the round began), so n + 1 rounds end
 // any run; more means a fault, reported as an error.
@@ -1276,7 +1276,7 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
   HIP_TRY(launch_mis_init<OffT>(a, grid, c->stream));
   const int64_t limit = g->n + 2;
   int64_t launched = 0;
-  int K = 16;
+  const int K = 8;
   for (;;) {
     const int k = (int)std::min<int64_t>(K, limit - launched);
     for (int i = 0; i < k; ++i) HIP_TRY(launch_mis_round<OffT>(a, step, grid, c->stream));
@@ -1288,7 +1288,6 @@ static int run_mis(pbbs_ctx* c, pbbs_graph* g, int32_t* d_out, pbbs_mis_stats* s
     if (launched >= limit)
       return fail(PBBS_ERR_STATE, "set rounds did not finish in %lld launches",
                   (long long)launched);
-    K = 8;
   }
   const int32_t rounds = hs[1];
   HIP_TRY(launch_mis_output<OffT>(a, d_out, c->s_ctl + kMisCtlMembers, grid, c->stream));

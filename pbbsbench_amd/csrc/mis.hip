@@ -208,6 +208,59 @@ __global__ __launch_bounds__(kBlock) void mis_round_kernel(MisArgs<OffT> a) {
       }
       next += (unsigned long long)__popcll(free_m);
     }
+    // Look again at the chunk's deferred vertices (in LDS; two passes on
+    // graphs read 4 entries a look, one on sparser ones, by measurement):
+    // by now their blockers have had a chunk's time to decide, and their rows
+    // and flags were just read, so a second look mostly hits in cache where
+    // a look in the next round would not.  Vertices still blocked are
+    // compacted to the front of the buffer (writes land below the read
+    // cursor: a pass never writes more entries than it has taken).
+    constexpr int kRetry = kStep == 4 ? 2 : 1;
+    for (int pass = 0; pass < kRetry && ndefer; ++pass) {
+      __builtin_amdgcn_wave_barrier();
+      const unsigned total = ndefer;
+      unsigned nd2 = 0, nxt = kWave;
+      bool hv = lane < total;
+      int32_t w = 0;
+      OffT q = 0, wlo = 0, whi = 0;
+      if (hv) {
+        w = dbuf[lane];
+        wlo = a.off[w];
+        whi = a.off[w + 1];
+        q = wlo + (OffT)a.pos[w];
+      }
+      while (__ballot(hv)) {
+        int res = -1;
+        if (hv) {
+          bool blocked = false;
+          const int d = mis_look<kStep>(a, w, q, whi, blocked);
+          if (d != 0) res = d;
+          else if (blocked) res = 0;
+        }
+        if (res > 0) a.flags[w] = (uint8_t)res;
+        const unsigned long long dm = __ballot(res == 0);
+        if (res == 0) {
+          a.pos[w] = (uint32_t)(q - wlo);
+          dbuf[nd2 + lanes_below_mask(dm)] = w;
+        }
+        nd2 += (unsigned)__popcll(dm);
+        __builtin_amdgcn_wave_barrier();
+        const bool take = res >= 0 || !hv;
+        const unsigned long long fm = __ballot(take);
+        if (take) {
+          const unsigned i = nxt + lanes_below_mask(fm);
+          hv = i < total;
+          if (hv) {
+            w = dbuf[i];
+            wlo = a.off[w];
+            whi = a.off[w + 1];
+            q = wlo + (OffT)a.pos[w];
+          }
+        }
+        nxt += (unsigned)__popcll(fm);
+      }
+      ndefer = nd2;
+    }
     __builtin_amdgcn_wave_barrier();
     if (ndefer) {
       const unsigned long long t = k * kMisStripes / nchunks;
