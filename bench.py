@@ -855,6 +855,10 @@ def run_st(args):
             cnt, st = ctx.spanning_forest_device(n, e.m, e.u, e.v)
             dev.append(st["device_ms"])
         wall_ms = (time.perf_counter() - t0) / args.steps * 1e3
+        # ndST's contract (any spanning forest) on the same edges
+        ctx.spanning_forest_device(n, e.m, e.u, e.v, any_forest=True)
+        adev = [ctx.spanning_forest_device(n, e.m, e.u, e.v, any_forest=True)[1]["device_ms"]
+                for _ in range(args.steps)]
         u, v = e.download()
     finally:
         e.free()
@@ -865,6 +869,8 @@ def run_st(args):
         "ms_per_step": round(wall_ms, 4),
         "rounds": int(st["rounds"]), "forest_edges": int(cnt),
         "medges_per_s": round(u.size / (ev / 1e3) / 1e6, 1),
+        "any_forest": {"contract": "any spanning forest (spanningForest/ndST/ST.C)",
+                       "event_ms_per_forest": round(float(np.mean(adev)), 4)},
     }
     exact = True
     if args.cpu_seconds > 0:
@@ -875,6 +881,10 @@ def run_st(args):
         cpu_s = time.perf_counter() - t1
         same = bool(np.array_equal(f, want)) and rounds == st["rounds"]
         exact = same and oracle.check_st(n, u, v, f) == 0
+        fa, _ = ctx.spanning_forest(n, u, v, any_forest=True)
+        any_ok = oracle.check_st(n, u, v, fa) == 0
+        out["any_forest"]["check"] = "STCheck rule holds" if any_ok else "INVALID"
+        exact = exact and any_ok
         out["cpu_baseline"] = {
             "value": round(cpu_s * 1e3, 1), "unit": "ms per forest", "cores": 1, "kind": "port",
             "sample": ("one forest on the same edges by the oracle's serial C restatement of "

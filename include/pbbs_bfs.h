@@ -224,6 +224,12 @@ int pbbs_mis(pbbs_ctx* ctx, pbbs_graph* g, int32_t* flags, uint32_t out_flags,
 
 /* ---- spanning forest ---------------------------------------------------- */
 #define PBBS_EDGES_DEVICE 0x2u   /* the edge arrays are device pointers */
+/* Any spanning forest (the contract of spanningForest/ndST/ST.C, checked by
+ * bench/STCheck): one pass of hook-and-link over the edges instead of
+ * incrementalST's round schedule; which forest, and its order, vary from
+ * run to run.  Ids are listed by the hooked root, as ndST filters its hooks.
+ * No m < 300 refusal (there is no schedule). */
+#define PBBS_ST_ANY_FOREST 0x8u
 
 typedef struct {
     int32_t rounds;         /* rounds of the speculative schedule */

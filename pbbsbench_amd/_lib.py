@@ -160,6 +160,7 @@ class ServerResult(C.Structure):
 
 PBBS_CHECK_DEVICE_INPUT = 0x1
 PBBS_EDGES_DEVICE = 0x2
+PBBS_ST_ANY_FOREST = 0x8
 
 
 class CheckResult(C.Structure):

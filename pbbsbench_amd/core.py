@@ -313,10 +313,11 @@ class Context:
                                    st), "BFS multi")
         return [st[j].as_dict() for j in range(J)]
 
-    def spanning_forest(self, n, u, v):
+    def spanning_forest(self, n, u, v, any_forest=False):
         """Spanning forest of the edge array (u[i], v[i]) on n vertices: the edge
         ids incrementalST returns (spanningForest/incrementalST/ST.C:35-71), in
-        its order.  Returns (ids int32[k], stats dict)."""
+        its order; any_forest: some spanning forest (ndST's contract,
+        spanningForest/ndST/ST.C).  Returns (ids int32[k], stats dict)."""
         eu = np.ascontiguousarray(u, dtype=np.int32)
         ev = np.ascontiguousarray(v, dtype=np.int32)
         if eu.shape != ev.shape:
@@ -324,17 +325,19 @@ class Context:
         out = np.empty(max(int(n) - 1, 1), dtype=np.int32)
         cnt = C.c_int64()
         st = _lib.StStats()
-        check(lib().pbbs_spanning_forest(self._h, int(n), eu.size, _ptr(eu), _ptr(ev), 0,
+        flags = _lib.PBBS_ST_ANY_FOREST if any_forest else 0
+        check(lib().pbbs_spanning_forest(self._h, int(n), eu.size, _ptr(eu), _ptr(ev), flags,
                                          _ptr(out), C.byref(cnt), C.byref(st)), "spanning_forest")
         return out[:cnt.value].copy(), st.as_dict()
 
-    def spanning_forest_device(self, n, m, u_ptr, v_ptr, forest_ptr=None):
+    def spanning_forest_device(self, n, m, u_ptr, v_ptr, forest_ptr=None, any_forest=False):
         """spanning_forest() on edge arrays already in HBM (device pointers to m
         int32 each); the ids stay in HBM (forest_ptr, room for n - 1, or the
         context's own buffer when None).  Returns (count, stats dict)."""
         cnt = C.c_int64()
         st = _lib.StStats()
-        flags = _lib.PBBS_EDGES_DEVICE | (_lib.PBBS_OUT_DEVICE if forest_ptr else 0)
+        flags = (_lib.PBBS_EDGES_DEVICE | (_lib.PBBS_OUT_DEVICE if forest_ptr else 0) |
+                 (_lib.PBBS_ST_ANY_FOREST if any_forest else 0))
         check(lib().pbbs_spanning_forest(self._h, int(n), int(m), u_ptr, v_ptr, flags,
                                          forest_ptr, C.byref(cnt), C.byref(st)),
               "spanning_forest")

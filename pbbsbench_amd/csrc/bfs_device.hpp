@@ -212,7 +212,8 @@ struct StCtl {
   long long scatter_size;  // the round just scheduled, for its pack kernel (0: none)
   long long rounds;        // rounds scheduled so far
   long long count;         // forest edges, after the output scan
-  int error;               // 1 an edge end outside [0, n) (bit), 2 the schedule gave up
+  int error;               // bits: 1 an edge end outside [0, n), 2 the schedule gave up,
+                           // 4 an any-forest edge ran out of tries
   int finished;
 };
 

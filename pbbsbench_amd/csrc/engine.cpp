@@ -64,6 +64,7 @@ hipError_t launch_mis_output(const MisArgs<OffT>& a, int32_t* flags_out,
                              unsigned long long* members, int grid, hipStream_t stream);
 hipError_t launch_st_init(const StArgs& a, int grid, hipStream_t stream);
 hipError_t launch_st_round(const StArgs& a, int grid, hipStream_t stream);
+hipError_t launch_st_any(const StArgs& a, int grid, hipStream_t stream);
 hipError_t launch_st_output(const StArgs& a, int32_t* forest, int grid, hipStream_t stream);
 template <typename OffT>
 hipError_t launch_pull_rows(const OffT* off, const int32_t* rows, long long nrows, long long m,
@@ -1346,14 +1347,15 @@ extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int
                                     int64_t* count, pbbs_st_stats* stats) {
   clear_error();
   if (!c || !count) return fail(PBBS_ERR_INVALID, "NULL context or count");
-  if (flags & ~(PBBS_OUT_DEVICE | PBBS_EDGES_DEVICE))
+  if (flags & ~(PBBS_OUT_DEVICE | PBBS_EDGES_DEVICE | PBBS_ST_ANY_FOREST))
     return fail(PBBS_ERR_INVALID, "unknown flags 0x%x", flags);
+  const bool any = (flags & PBBS_ST_ANY_FOREST) != 0;
   if (n <= 0 || n > INT32_MAX) return fail(PBBS_ERR_LIMIT, "n=%lld outside [1, 2^31)", (long long)n);
   if (m < 0 || m >= (int64_t)UINT32_MAX)
     return fail(PBBS_ERR_LIMIT, "m=%lld outside [0, 2^32 - 1)", (long long)m);
   if (m > 0 && (!u || !v)) return fail(PBBS_ERR_INVALID, "NULL edge array");
   const int64_t max_round = m / kStGranularity + 1;
-  if (m > 0 && max_round / 4 == 0)
+  if (!any && m > 0 && max_round / 4 == 0)
     return fail(PBBS_ERR_LIMIT,
                 "m=%lld < %lld: the reference's round schedule starts at size 0 and gives up "
                 "(common/speculative_for.h:36-40)",
@@ -1408,7 +1410,13 @@ extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int
   const int64_t limit = a.max_tries + 2;
   int64_t launched = 0;
   int K = (int)std::min<int64_t>(std::max<int64_t>((m + max_round - 1) / max_round, 8), limit);
-  for (;;) {
+  if (any) {   // one pass over the edges, no schedule
+    const int any_grid = (int)std::min<int64_t>(std::max<int64_t>((m + kBlock - 1) / kBlock, 1),
+                                                16 * c->num_cus);
+    if (m > 0) HIP_TRY(launch_st_any(a, any_grid, c->stream));
+    K = 0;
+  }
+  for (; K > 0;) {
     const int k = (int)std::min<int64_t>(K, limit - launched);
     for (int i = 0; i < k; ++i) HIP_TRY(launch_st_round(a, grid, c->stream));
     launched += k;
@@ -1430,6 +1438,7 @@ extern "C" int pbbs_spanning_forest(pbbs_ctx* c, int64_t n, int64_t m, const int
                                 (long long)n);
   if (hc.error & 2) return fail(PBBS_ERR_STATE, "the round schedule gave up after %lld rounds",
                                 (long long)hc.rounds);
+  if (hc.error & 4) return fail(PBBS_ERR_STATE, "an edge's hook did not settle");
   float dev_ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&dev_ms, c->ev_start, c->ev_end));
   if (!dev_edges) {

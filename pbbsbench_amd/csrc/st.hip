@@ -241,6 +241,78 @@ __global__ __launch_bounds__(kBlock) void st_pack_kernel(StArgs a) {
   }
 }
 
+// The ndST contract (spanningForest/ndST/ST.C): any spanning forest.  One
+// pass over the edges: an edge finds the roots u < v of its ends and, if it
+// is the first to claim hooks[u] (compare-and-swap from free), links u under
+// v; a lost claim or a stale root means finding again.  Links go from the
+// smaller root to the larger, so no cycle forms; each root is hooked once.
+// Finds here run beside links, so they read parents at agent scope (not a
+// stale L1 copy).  A claim lost to an edge whose link is not yet visible is
+// retried; kAnyTries bounds that wait, and running out marks the run as
+// failed (error bit 4) rather than spinning on.
+constexpr int kAnyTries = 1 << 16;
+
+__device__ __forceinline__ int32_t ld_parent(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int32_t st_find_any(int32_t* parents, int32_t i) {
+  int32_t p = ld_parent(&parents[i]);
+  if (p < 0) return i;
+  int32_t gp = ld_parent(&parents[p]);
+  while (gp >= 0) {
+    __hip_atomic_store(&parents[i], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    i = p;
+    p = gp;
+    gp = ld_parent(&parents[p]);
+  }
+  return p;
+}
+
+__global__ __launch_bounds__(kBlock) void st_any_kernel(StArgs a) {
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nth = (long long)gridDim.x * blockDim.x;
+  uint32_t* hooks = a.R;
+  for (long long i = tid; i < a.m; i += nth) {
+    int32_t u = a.eu[i], v = a.ev[i];
+    if ((uint32_t)u >= (uint32_t)a.n || (uint32_t)v >= (uint32_t)a.n) {
+      atomicOr(&a.ctl->error, 1);
+      continue;
+    }
+    // The retry loop runs while any lane of the wavefront still retries, and
+    // a winning lane links inside it.  With a per-lane exit the compiler may
+    // move the link store to the loop's exit, which the wavefront reaches
+    // only when every lane is done: a lane waiting for that link would then
+    // never see it.
+    bool active = true;
+    int t = 0;
+    while (__ballot(active)) {
+      if (active) {
+        u = st_find_any(a.parents, u);
+        v = st_find_any(a.parents, v);
+        if (u == v) {
+          active = false;
+        } else {
+          if (u > v) {
+            const int32_t x = u;
+            u = v;
+            v = x;
+          }
+          if (__hip_atomic_load(&hooks[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                  UINT_MAX &&
+              atomicCAS(&hooks[u], UINT_MAX, (uint32_t)i) == UINT_MAX) {
+            __hip_atomic_store(&a.parents[u], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            active = false;
+          } else if (++t == kAnyTries) {
+            atomicOr(&a.ctl->error, 4);
+            active = false;
+          }
+        }
+      }
+    }
+  }
+}
+
 // filter_map of R (ST.C:67-69): per-workgroup counts of reserved entries,
 // one workgroup's scan (st_schedule_kernel's scan with `out_scan`), then the
 // ordered writes.
@@ -335,6 +407,11 @@ hipError_t launch_st_round(const StArgs& a, int grid, hipStream_t stream) {
   hipLaunchKernelGGL(st_commit_kernel, dim3(grid), dim3(kBlock), 0, stream, a);
   hipLaunchKernelGGL(st_schedule_kernel, dim3(1), dim3(1024), 0, stream, a, grid);
   hipLaunchKernelGGL(st_pack_kernel, dim3(grid), dim3(kBlock), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_st_any(const StArgs& a, int grid, hipStream_t stream) {
+  hipLaunchKernelGGL(st_any_kernel, dim3(grid), dim3(kBlock), 0, stream, a);
   return hipGetLastError();
 }
 

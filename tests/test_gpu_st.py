@@ -184,3 +184,41 @@ def test_graph_edges_on_gpu(ctx):
         for e in es:
             e.free()
         dg.free()
+
+
+def test_st_any_forest(ctx):
+    """PBBS_ST_ANY_FOREST (ndST's contract, spanningForest/ndST/ST.C): some
+    spanning forest, checked by STCheck's rule (bench/STCheck.C:36-61) and by
+    its size against the serial forest; ids are listed by hooked root, so
+    they are distinct and each hooks a different vertex."""
+    rng = np.random.default_rng(12)
+    cases = [(50, rng.integers(0, 50, 120), rng.integers(0, 50, 120)),       # m < 300 is fine here
+             (5000, rng.integers(0, 5000, 40000), rng.integers(0, 5000, 40000)),
+             (2, np.zeros(5000, np.int64), np.ones(5000, np.int64))]
+    g = pb.rmat(16, 12 * 2 ** 16, 0.55, 0.125, 0.125, seed=3, permute_seed=1)
+    u, v = graph_edges(g, 4)
+    cases.append((g.n, u, v))
+    t = pb.grid3d(24)
+    u, v = graph_edges(t, 5)
+    cases.append((t.n, u, v))
+    for n, u, v in cases:
+        u = np.asarray(u, np.int32)
+        v = np.asarray(v, np.int32)
+        say(f"any forest n={n} m={u.size}")
+        f, st = ctx.spanning_forest(n, u, v, any_forest=True)
+        assert oracle.check_st(n, u, v, f) == 0
+        assert f.size == oracle.serial_st(n, u, v).size == st["edges"]
+        assert np.unique(f).size == f.size
+    with pytest.raises(pb.PbbsError, match="outside"):
+        ctx.spanning_forest(5, np.zeros(10, np.int32), np.full(10, 7, np.int32), any_forest=True)
+
+
+def test_st_any_forest_rmat22():
+    with pb.Context(0) as c:
+        g = pb.rmat(22, 12 * 2 ** 22, 0.55, 0.125, 0.125, seed=1, permute_seed=2)
+        u, v = graph_edges(g, 7)
+        del g
+        say("any forest rmat22")
+        f, st = c.spanning_forest(2 ** 22, u, v, any_forest=True)
+        assert oracle.check_st(2 ** 22, u, v, f) == 0
+        say(f"edges={st['edges']} ms={st['device_ms']:.2f}")
