@@ -1,6 +1,8 @@
 """MI355X (gfx950) deterministic breadth-first search, a drop-in for PBBS
-breadthFirstSearch/deterministicBFS.  The work runs in libpbbs_bfs.so (HIP
-kernels + host engine) behind the C ABI of include/pbbs_bfs.h."""
+breadthFirstSearch/deterministicBFS, with its siblings on the same graphs
+(maximalIndependentSet's incrementalMIS, spanningForest's incrementalST).  The
+work runs in libpbbs_bfs.so (HIP kernels + host engine) behind the C ABI of
+include/pbbs_bfs.h."""
 from ._lib import PbbsError, LIB_PATH  # noqa: F401
 from .core import (  # noqa: F401
     BFS,
