@@ -759,6 +759,30 @@ def bench_ingest(args):
     print(json.dumps(result))
 
 
+def mis_traffic():
+    """HBM bytes per set of the rounds and the output, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_mis_rmat24jr.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic_mis_rmat24jr.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return (t["round_fetch_bytes_per_set"] + t["round_write_bytes_per_set"] +
+            t["output_fetch_bytes_per_set"] + t["output_write_bytes_per_set"])
+
+
+def st_traffic():
+    """HBM bytes per forest of the reserve and commit steps, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_st_rmat24e.json)."""
+    path = os.path.join(ROOT, "profiles", "traffic_st_rmat24e.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return (t["reserve_fetch_bytes_per_forest"] + t["reserve_write_bytes_per_forest"] +
+            t["commit_fetch_bytes_per_forest"] + t["commit_write_bytes_per_forest"])
+
+
 MIS_GRAPH = ("rMatGraph_JR_12_16000000 (one of the three graphs the maximal independent set "
              "benchmark times, maximalIndependentSet/bench/testInputs): 2^24 vertices, 12 * 16M "
              "edge draws, a=.55 b=c=.125, relabelled by a seeded permutation, built in HBM")
@@ -797,10 +821,15 @@ def run_mis(args):
         "d2h_ms": round(st_host["copy_ms"], 3),
         "check": {"valid": chk["valid"], "bad_edges": chk["bad_edges"],
                   "bad_vertices": chk["bad_vertices"]},
-        "graph_pass_gbs": round(B / (ev / 1e3) / 1e9, 1),
-        "graph_pass_frac": round(B / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-        "graph_pass_note": ("bytes of one pass over offsets, rows and flags (the set reads row "
-                            "prefixes only, so this normalises, it is not a traffic count)"),
+        "roofline": {
+            "bound": "hbm", "kernel": "mis_init + mis_round per round + mis_output (one set)",
+            "achieved": round(B / (ev / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(B / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": int(B),
+            "algorithmic_note": ("one pass over offsets, rows and flags, plus the int32 output "
+                                 "(the set reads row prefixes only, so this normalises)"),
+            "traffic": mis_traffic(),
+        },
     }
     exact = chk["valid"] == 1
     if args.cpu_seconds > 0:
@@ -863,12 +892,22 @@ def run_st(args):
     finally:
         e.free()
     ev = float(np.mean(dev))
+    Bst = 8 * int(u.size) + 12 * n   # each edge read once; parents, R and output once per vertex
     out = {
         "graph": ST_GRAPH, "n": n, "m": int(u.size), "gen_s": round(gen_s, 3),
         "event_ms_per_forest": round(ev, 4),
         "ms_per_step": round(wall_ms, 4),
         "rounds": int(st["rounds"]), "forest_edges": int(cnt),
         "medges_per_s": round(u.size / (ev / 1e3) / 1e6, 1),
+        "roofline": {
+            "bound": "hbm", "kernel": "st_reserve + st_commit + st_schedule + st_pack per round",
+            "achieved": round(Bst / (ev / 1e3) / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(Bst / (ev / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+            "algorithmic_bytes": Bst,
+            "algorithmic_note": ("each edge read once and each vertex's parent, reservation and "
+                                 "output once; the schedule's finds re-read parents at random"),
+            "traffic": st_traffic(),
+        },
         "any_forest": {"contract": "any spanning forest (spanningForest/ndST/ST.C)",
                        "event_ms_per_forest": round(float(np.mean(adev)), 4)},
     }
