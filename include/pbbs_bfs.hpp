@@ -10,6 +10,8 @@
 // this side of the boundary only.
 #pragma once
 
+#include <algorithm>
+#include <cstdint>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -113,6 +115,47 @@ inline std::vector<vertexId> BFS(vertexId start, const Graph& G, bool verbose = 
                  stats),
         "BFS");
   return parent;
+}
+
+// Maximal independent set of G, as maximalIndependentSet(G) returns it
+// (maximalIndependentSet/bench/MIS.h, incrementalMIS/MIS.C:62-66): one char
+// per vertex, 1 in the set, 2 not.
+inline std::vector<char> maximalIndependentSet(const Graph& G, pbbs_mis_stats* stats = nullptr) {
+  std::vector<int32_t> f(G.numVertices());
+  check(pbbs_mis(G.ctx(), G.get(), f.data(), 0, stats), "maximalIndependentSet");
+  return std::vector<char>(f.begin(), f.end());
+}
+
+// Edge array (u[i], v[i]) on n vertices, as edgeArray<vertexId> holds it
+// (common/graph.h), read with readEdgeArrayFromFile (common/graphIO.h:147-166).
+struct EdgeArray {
+  int64_t n = 0;
+  std::vector<int32_t> u, v;
+  static EdgeArray read(const char* path) {
+    int64_t n = 0, m = 0;
+    int32_t *pu = nullptr, *pv = nullptr;
+    check(pbbs_read_edge_array(path, &n, &m, &pu, &pv), "readEdgeArrayFromFile");
+    EdgeArray e;
+    e.n = n;
+    e.u.assign(pu, pu + m);
+    e.v.assign(pv, pv + m);
+    pbbs_free(pu);
+    pbbs_free(pv);
+    return e;
+  }
+};
+
+// Spanning forest of E, as st(E) returns it (spanningForest/bench/ST.h:7,
+// incrementalST/ST.C:62-71): the forest's edge ids.  `flags` may hold
+// PBBS_ST_ANY_FOREST (ndST's contract).
+inline std::vector<uint32_t> st(Context& ctx, const EdgeArray& E, pbbs_st_stats* stats = nullptr,
+                                 uint32_t flags = 0) {
+  std::vector<int32_t> ids((size_t)std::max<int64_t>(E.n - 1, 1));
+  int64_t count = 0;
+  check(pbbs_spanning_forest(ctx.get(), E.n, (int64_t)E.u.size(), E.u.data(), E.v.data(), flags,
+                             ids.data(), &count, stats),
+        "st");
+  return std::vector<uint32_t>(ids.begin(), ids.begin() + count);
 }
 
 }  // namespace pbbs_amd

@@ -97,3 +97,27 @@ def test_set_and_forest_drivers_usage_and_bad_files(tmp_path, name):
     bad.write_text("AdjacencyGraph\n2\n1\n0\n" if name == "ST" else "EdgeArray\n0 1\n")
     out = subprocess.run([exe, str(bad)], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "missing header" in out.stderr
+
+
+def test_cpp_header_compiles_with_every_wrapper(tmp_path):
+    """include/pbbs_bfs.hpp's wrappers (BFS, maximalIndependentSet, st) type-check
+    against the C ABI; nothing runs."""
+    src = tmp_path / "use.cpp"
+    src.write_text(
+        '#include "pbbs_bfs.hpp"\n'
+        "int main(int argc, char** argv) {\n"
+        "  if (argc < 3) return 0;\n"
+        "  pbbs_amd::Context ctx(0);\n"
+        "  pbbs_amd::HostGraph h = pbbs_amd::HostGraph::read(argv[1]);\n"
+        "  pbbs_amd::Graph g(ctx, h);\n"
+        "  std::vector<int> p = pbbs_amd::BFS(0, g);\n"
+        "  std::vector<char> f = pbbs_amd::maximalIndependentSet(g);\n"
+        "  pbbs_amd::EdgeArray e = pbbs_amd::EdgeArray::read(argv[2]);\n"
+        "  std::vector<uint32_t> t = pbbs_amd::st(ctx, e);\n"
+        "  std::vector<uint32_t> a = pbbs_amd::st(ctx, e, nullptr, PBBS_ST_ANY_FOREST);\n"
+        "  return (int)(p.size() + f.size() + t.size() + a.size()) & 1;\n"
+        "}\n")
+    out = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                          "-I", os.path.join(ROOT, "include"), str(src)],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
