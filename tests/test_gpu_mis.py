@@ -207,3 +207,23 @@ def test_check_mis_on_gpu(ctx):
             hip.hipFree(buf)
     finally:
         dg.free()
+
+
+def test_mis_full_size_randlocal_and_grid():
+    """randLocalGraph_JR_10_20000000's and 3Dgrid_JR_64000000's shapes at full
+    size (relabelled, rows sorted): the flags equal the oracle's, and the GPU
+    MISCheck accepts them."""
+    with pb.Context(0) as c:
+        for name, g in (("randlocal 20M", pb.permute(pb.randlocal(20_000_000, 10, seed=1), 3)),
+                        ("3D torus 400^3", pb.permute(pb.grid3d(400), 5))):
+            say(f"mis full size: {name} n={g.n} m={g.m}")
+            dg = c.upload(g, assume_symmetric=True)
+            try:
+                flags, st = dg.mis()
+                r = dg.check_mis(flags)
+            finally:
+                dg.free()
+            say(f"rounds={st['rounds']} ms={st['device_ms']:.2f}")
+            assert r["valid"] == 1, r
+            assert np.array_equal(flags, oracle.mis(g.offsets, g.adj))
+            del g

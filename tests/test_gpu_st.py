@@ -222,3 +222,23 @@ def test_st_any_forest_rmat22():
         f, st = c.spanning_forest(2 ** 22, u, v, any_forest=True)
         assert oracle.check_st(2 ** 22, u, v, f) == 0
         say(f"edges={st['edges']} ms={st['device_ms']:.2f}")
+
+
+def test_st_full_size_rmat24_edges():
+    """The spanning-forest bench input at full size (190.6M shuffled RMAT
+    edges, built in HBM): ids and round count equal the oracle's."""
+    with pb.Context(0) as c:
+        dg = c.gen_rmat(24, 12 * 16_000_000, 0.55, 0.125, 0.125, seed=1, permute_seed=2)
+        n = dg.n
+        e = dg.edges_device(shuffle_seed=3)
+        dg.free()
+        try:
+            say(f"spanning forest full size m={e.m}")
+            cnt, st = c.spanning_forest_device(n, e.m, e.u, e.v)
+            u, v = e.download()
+        finally:
+            e.free()
+        f, st2 = c.spanning_forest(n, u, v)
+        want, rounds = oracle.st(n, u, v)
+        assert cnt == f.size and st["rounds"] == st2["rounds"] == rounds
+        assert np.array_equal(f, want)
